@@ -1,0 +1,215 @@
+/*
+ * qm_hip.h -- C ABI of libqmhip.so, the MI355X (gfx950) state-vector backend
+ * for QuantumMatter.
+ *
+ * Plain C types only: pointers, sizes, POD records.  No C++ exceptions cross
+ * this boundary.  Every entry point returns a QM_* status; qm_last_error()
+ * returns a thread-local message for the last failing call on that thread.
+ * The caller owns every host buffer; the library keeps no host pointer after
+ * a call returns.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the QuantumMatter repository).  The C# P/Invoke binding that a
+ * QuantumMatter.Infrastructure maintainer adds is given in INTEGRATION.md.
+ */
+#ifndef QM_HIP_H
+#define QM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QM_ABI_VERSION 1
+
+/* Status codes.  The shim maps them to the .NET exceptions the reference
+ * raises for the same condition. */
+#define QM_OK 0
+#define QM_ERR_ARG 1         /* ArgumentException / ArgumentNullException */
+#define QM_ERR_RANGE 2       /* ArgumentOutOfRangeException */
+#define QM_ERR_MASS 3        /* InvalidOperationException: no finite positive mass */
+#define QM_ERR_NONFINITE 4   /* InvalidOperationException: nonfinite accumulation */
+#define QM_ERR_POOL 5        /* uniform pool exhausted (the System.Random stand-in) */
+#define QM_ERR_UNSUPPORTED 6 /* NotSupportedException */
+#define QM_ERR_BUDGET 7      /* InvalidOperationException: device/staging budget */
+#define QM_ERR_CANCELLED 8   /* OperationCanceledException */
+#define QM_ERR_NODEVICE 9    /* PlatformNotSupportedException */
+#define QM_ERR_DISPOSED 10   /* ObjectDisposedException */
+#define QM_ERR_DEVICE 11     /* HIP runtime failure */
+#define QM_ERR_STATE 12      /* InvalidOperationException (API misuse, e.g. profiling off) */
+
+/* Amplitude precision of a device state. */
+#define QM_C128 0 /* complex128: interleaved (re,im) doubles, 16 B per amplitude */
+#define QM_C64 1  /* complex64: interleaved (re,im) floats, 8 B; FP64 reductions */
+
+/* Execution schedules for qm_state_execute. */
+#define QM_SCHED_PER_GATE 0 /* one launch per non-identity gate (the reference's CUDA shape) */
+#define QM_SCHED_FUSED 1    /* register/LDS tile passes: many gates per HBM pass (default) */
+
+/* GateOperation (src/QuantumMatter.Core/QuantumCircuit.cs:49-61).  kind uses
+ * the GateKind numbering (:4-40); control/second_control are -1 when absent;
+ * Swap(a,b) stores control=a, target=b; unitary is the row-major 2x2 matrix
+ * m00,m01,m10,m11 as (re,im) pairs, read only for kind 15 (Unitary). */
+typedef struct qm_op {
+  int32_t kind;
+  int32_t target;
+  int32_t control;
+  int32_t second_control;
+  double angle;
+  double unitary[8];
+} qm_op;
+
+/* PauliString masks (src/QuantumMatter.Core/PauliString.cs:15-26). */
+typedef struct qm_pauli {
+  uint64_t flip_mask;
+  uint64_t sign_mask;
+  int32_t y_count;
+  int32_t pad;
+} qm_pauli;
+
+/* One NoisyCircuit step (src/QuantumMatter.Core/Noise/NoisyCircuit.cs:48-65):
+ * a unitary segment ops[op_begin, op_begin+op_count) or, when is_channel, a
+ * NoiseKind channel (Noise/NoiseChannel.cs:4-13) of parameter `probability`. */
+typedef struct qm_step {
+  int32_t is_channel;
+  int32_t noise_kind;
+  int32_t target;
+  int32_t pad;
+  double probability;
+  int64_t op_begin;
+  int64_t op_count;
+} qm_step;
+
+/* CudaDeviceInfo (src/QuantumMatter.Infrastructure/CudaBackendOptions.cs:29). */
+typedef struct qm_device_info {
+  int32_t index;
+  int32_t compute_units;
+  int64_t memory_bytes;
+  char name[128];
+  char arch[64];
+  char driver[64];
+} qm_device_info;
+
+/* CudaBackendOptions (CudaBackendOptions.cs:4-20).  Budgets >= 32 bytes. */
+typedef struct qm_ctx_options {
+  int32_t device_index;
+  int32_t enable_profiling;
+  int64_t device_budget_bytes;
+  int64_t staging_budget_bytes;
+} qm_ctx_options;
+
+/* CudaExecutionProfile (CudaBackendOptions.cs:35) plus the schedule's shape. */
+typedef struct qm_profile {
+  double device_ms;          /* hipEvent interval around the gate sequence */
+  int64_t state_bytes;       /* bytes of the device state */
+  int32_t kernel_launches;   /* launches issued for the sequence */
+  int32_t passes;            /* full-state HBM passes issued */
+  double algorithmic_bytes;  /* bytes the executed schedule must move */
+} qm_profile;
+
+typedef struct qm_ctx qm_ctx;
+typedef struct qm_state qm_state;
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char *qm_last_error(void);
+int32_t qm_abi_version(void);
+
+/* ---- devices and contexts ---------------------------------------------- */
+/* CudaStateVectorBackend.GetDevices (CudaStateVectorBackend.cs:70-85). */
+int qm_device_count(int32_t *count);
+int qm_device_get_info(int32_t index, qm_device_info *out);
+
+/* CudaStateVectorBackend ctor (CudaStateVectorBackend.cs:21-64): validates
+ * options, QM_ERR_NODEVICE without a device, QM_ERR_RANGE for a bad index. */
+int qm_ctx_create(const qm_ctx_options *options, qm_ctx **out);
+/* CudaStateVectorBackend.Dispose (:191-216): waits for work; idempotent per
+ * handle value is the caller's concern (the shim keeps a disposed flag). */
+int qm_ctx_destroy(qm_ctx *ctx);
+/* Device memory currently held by this context's states (bytes). */
+int qm_ctx_resident_bytes(qm_ctx *ctx, int64_t *bytes);
+
+/* ---- drop-in whole-circuit execution ------------------------------------ */
+/* IStateVectorBackend.Execute (src/QuantumMatter.Core/IStateVectorBackend.cs:13)
+ * as implemented by CudaStateVectorBackend.ExecuteCore (:103-188): validates
+ * n_amplitudes == 2^qubits, checks budgets before allocation, uploads,
+ * evolves in insertion order, downloads and commits into `amplitudes` only
+ * after success.  On any failure or cancellation the input is unchanged.
+ * cancel (nullable) is polled between launches.  profile (nullable) requires
+ * enable_profiling (else QM_ERR_STATE, like ExecuteProfiled :92-101). */
+int qm_execute(qm_ctx *ctx, int32_t qubits, const qm_op *ops, int64_t n_ops, double *amplitudes,
+               int64_t n_amplitudes, const volatile int32_t *cancel, qm_profile *profile);
+
+/* ---- device-resident states ---------------------------------------------- */
+/* StateVector ctor (src/QuantumMatter.Core/StateVector.cs:14-25) on the
+ * device: |0...0>, 1 <= qubits <= 34, 64-bit indexing.  Checked against the
+ * context's device budget before allocation. */
+int qm_state_create(qm_ctx *ctx, int32_t qubits, int32_t precision, qm_state **out);
+int qm_state_destroy(qm_state *state);
+int qm_state_info(qm_state *state, int32_t *qubits, int32_t *precision);
+/* StateVector.Reset (StateVector.cs:66-71). */
+int qm_state_reset(qm_state *state);
+/* Host <-> device copies of interleaved (re,im) doubles; a C64 state rounds on
+ * upload and widens on download. */
+int qm_state_upload(qm_state *state, const double *amplitudes, int64_t n_amplitudes);
+int qm_state_download(qm_state *state, double *amplitudes, int64_t n_amplitudes);
+/* Copy (and convert precision) between states of equal width. */
+int qm_state_copy(qm_state *dst, qm_state *src);
+
+/* StateVector.Evolve (StateVector.cs:91-102) on the resident state. */
+int qm_state_execute(qm_state *state, const qm_op *ops, int64_t n_ops, int32_t schedule,
+                     const volatile int32_t *cancel, qm_profile *profile);
+
+/* INoiseChannelBackend.Apply (src/QuantumMatter.Core/Noise/INoiseChannelBackend.cs:13)
+ * as CpuNoiseChannelBackend.Apply (src/QuantumMatter.Infrastructure/CpuNoiseChannelBackend.cs:10-145).
+ * Draws are taken from uniforms[*cursor ...] under the reference's rules. */
+int qm_state_noise(qm_state *state, int32_t kind, double probability, int32_t target,
+                   const double *uniforms, int64_t n_uniforms, int64_t *cursor);
+/* StateVector.EvolveTrajectory -> NoisyCircuit.Execute (NoisyCircuit.cs:48-65). */
+int qm_state_noisy_execute(qm_state *state, const qm_step *steps, int64_t n_steps,
+                           const qm_op *ops, const double *uniforms, int64_t n_uniforms,
+                           int64_t *cursor);
+
+/* Born mass sum |a|^2 (StateVector.cs:109-124), FP64 on device. */
+int qm_state_mass(qm_state *state, double *mass);
+/* StateVector.MeasureAll (StateVector.cs:106-153) with draw u. */
+int qm_state_measure_all(qm_state *state, double u, int64_t *outcome);
+/* StateVector.Measure (StateVector.Observations.cs:45-102). */
+int qm_state_measure(qm_state *state, const int32_t *qubits, int32_t k, double u,
+                     int64_t *outcome);
+/* StateVector.MarginalProbability (StateVector.Observations.cs:10-36). */
+int qm_state_marginal(qm_state *state, const int32_t *qubits, int32_t k, int64_t outcome,
+                      double *probability);
+/* StateVectorSimulator.Run terminal sampling (src/QuantumMatter.Application/StateVectorSimulator.cs:44-90):
+ * one draw per shot, strict-upper-bound CDF search, no collapse. */
+int qm_state_sample(qm_state *state, const double *uniforms, int64_t shots, int64_t *outcomes);
+/* StateVector.ExpectationValue (StateVector.Observations.cs:107-141) for T
+ * strings at once; terms sharing a flip mask share one state read. */
+int qm_state_expectations(qm_state *state, const qm_pauli *terms, int64_t n_terms,
+                          double *values);
+/* StateVector.ExpectationEnergy (src/QuantumMatter.Core/StateVector.Energy.cs:10-46). */
+int qm_state_energy(qm_state *state, const qm_pauli *terms, const double *coefficients,
+                    int64_t n_terms, double *energy);
+/* <a|b> over two states of equal width, FP64 accumulation (re, im). */
+int qm_state_inner(qm_state *a, qm_state *b, double *re, double *im);
+
+/* ---- schedule inspection (host only, no device needed) -------------------- */
+/* No reference counterpart (the reference has no fusion, docs/gpu-execution.md:108):
+ * reports the fused schedule QM_SCHED_FUSED would run for these operations. */
+int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n_ops,
+                  int32_t *passes, int32_t *transposes, int64_t *gates);
+/* Runs the fused schedule for an arbitrary tile shape on the host with the
+ * kernels' exact arithmetic and indexing (complex128, qubits <= 26), so the
+ * planner is verified without a device. */
+int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
+                       int32_t lane_bits, int32_t contiguous, double *amplitudes);
+
+/* Decisions on CDF and branch boundaries: draws within the rigorous rounding
+ * margin of a boundary were re-decided with the reference's sequential
+ * summation order.  Returns the count accumulated on this state. */
+int qm_state_boundary_fallbacks(qm_state *state, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

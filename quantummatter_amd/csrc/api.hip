@@ -1,0 +1,832 @@
+// C ABI of libqmhip.so (include/qm_hip.h): contexts, resident states and the
+// host runtime around the kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "qm_internal.hpp"
+
+namespace qm {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+void clear_error() { g_error.clear(); }
+
+void fail(int code, const std::string &msg) { throw Status{code, msg}; }
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+      fail(QM_ERR_BUDGET, std::string("device allocation failed: ") + what);
+    fail(QM_ERR_DEVICE, std::string(hipGetErrorString(e)) + " in " + what);
+  }
+}
+
+template <typename Fn>
+static int guarded(Fn &&fn) {
+  try {
+    clear_error();
+    fn();
+    return QM_OK;
+  } catch (const Status &s) {
+    set_error(s.msg);
+    return s.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return QM_ERR_BUDGET;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return QM_ERR_DEVICE;
+  } catch (...) {
+    set_error("unknown failure");
+    return QM_ERR_DEVICE;
+  }
+}
+
+double *ctx_partials(Ctx *c, int64_t n) {
+  if (n > c->partial_cap) {
+    if (c->d_partial) {
+      QM_HIP(hipStreamSynchronize(c->stream));
+      QM_HIP(hipFree(c->d_partial));
+      c->d_partial = nullptr;
+    }
+    int64_t cap = std::max<int64_t>(n, 1 << 16);
+    QM_HIP(hipMalloc(&c->d_partial, (size_t)cap * sizeof(double)));
+    c->partial_cap = cap;
+  }
+  return c->d_partial;
+}
+
+int64_t *ctx_ipartials(Ctx *c, int64_t n) {
+  if (n > c->ipartial_cap) {
+    if (c->d_ipartial) {
+      QM_HIP(hipStreamSynchronize(c->stream));
+      QM_HIP(hipFree(c->d_ipartial));
+      c->d_ipartial = nullptr;
+    }
+    int64_t cap = std::max<int64_t>(n, 1 << 14);
+    QM_HIP(hipMalloc(&c->d_ipartial, (size_t)cap * sizeof(int64_t)));
+    c->ipartial_cap = cap;
+  }
+  return c->d_ipartial;
+}
+
+double *ctx_pinned(Ctx *c, int64_t n) {
+  if (n > c->pinned_cap) {
+    if (c->h_pinned) {
+      QM_HIP(hipStreamSynchronize(c->stream));
+      QM_HIP(hipHostFree(c->h_pinned));
+      c->h_pinned = nullptr;
+    }
+    int64_t cap = std::max<int64_t>(n, 4096);
+    QM_HIP(hipHostMalloc((void **)&c->h_pinned, (size_t)cap * sizeof(double), hipHostMallocDefault));
+    c->pinned_cap = cap;
+  }
+  return c->h_pinned;
+}
+
+void *ctx_program(Ctx *c, int64_t bytes) {
+  if (bytes > c->prog_cap) {
+    if (c->d_prog) {
+      QM_HIP(hipStreamSynchronize(c->stream));
+      QM_HIP(hipFree(c->d_prog));
+      c->d_prog = nullptr;
+    }
+    int64_t cap = std::max<int64_t>(bytes, 1 << 20);
+    QM_HIP(hipMalloc(&c->d_prog, (size_t)cap));
+    c->prog_cap = cap;
+  }
+  return c->d_prog;
+}
+
+static void use_device(Ctx *c) {
+  if (c == nullptr) fail(QM_ERR_ARG, "context is null");
+  if (c->disposed) fail(QM_ERR_DISPOSED, "The HIP backend has been disposed.");
+  QM_HIP(hipSetDevice(c->device));
+}
+
+static void check_cancel(const volatile int32_t *cancel) {
+  if (cancel && *cancel) fail(QM_ERR_CANCELLED, "The operation was canceled.");
+}
+
+static std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
+  if (n_ops < 0 || (n_ops > 0 && ops == nullptr)) fail(QM_ERR_ARG, "operations are null");
+  std::vector<DevGate> gates;
+  gates.reserve((size_t)n_ops);
+  for (int64_t i = 0; i < n_ops; i++) {
+    validate_op(ops[i], n);
+    DevGate g;
+    if (prepare_gate(ops[i], &g)) gates.push_back(g);
+  }
+  return gates;
+}
+
+struct ExecStats {
+  int launches = 0;
+  int passes = 0;
+  double bytes = 0;
+};
+
+// Runs prepared gates on a resident state with the chosen schedule.
+static ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule,
+                           const volatile int32_t *cancel) {
+  Ctx *c = s->ctx;
+  ExecStats st;
+  const double state_bytes = (double)s->bytes;
+  const FusedShape shape = fused_shape(s->precision);
+  if (schedule == QM_SCHED_FUSED && s->qubits >= shape.K && gates.size() > 1) {
+    FusedPlan plan = plan_fused(gates, s->qubits, shape);
+    std::vector<TileOp> flat = plan.flat_ops();
+    const int64_t pass_bytes = (int64_t)(plan.passes.size() * sizeof(DevPass));
+    const int64_t op_bytes = (int64_t)(flat.size() * sizeof(TileOp));
+    char *prog = (char *)ctx_program(c, pass_bytes + op_bytes + 256);
+    std::vector<char> host((size_t)(pass_bytes + op_bytes));
+    for (size_t i = 0; i < plan.passes.size(); i++)
+      std::memcpy(host.data() + i * sizeof(DevPass), &plan.passes[i].dev, sizeof(DevPass));
+    if (op_bytes) std::memcpy(host.data() + pass_bytes, flat.data(), (size_t)op_bytes);
+    QM_HIP(hipMemcpyAsync(prog, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
+    for (size_t i = 0; i < plan.passes.size(); i++) {
+      check_cancel(cancel);
+      launch_fused_pass(s, plan.passes[i], prog + i * sizeof(DevPass), prog + pass_bytes, c->stream);
+      st.launches++;
+      st.passes++;
+      st.bytes += 2 * state_bytes;
+      if (cancel && (st.launches & 31) == 0) QM_HIP(hipStreamSynchronize(c->stream));
+    }
+    // The program buffer must outlive the launches: wait before returning.
+    QM_HIP(hipStreamSynchronize(c->stream));
+    return st;
+  }
+  for (const DevGate &g : gates) {
+    check_cancel(cancel);
+    launch_gate(s, g, c->stream);
+    st.launches++;
+    st.passes++;
+    st.bytes += 2 * state_bytes * touched_fraction(g);
+    // Bound queued work and cancellation latency (CudaStateVectorBackend.cs:155-159).
+    if (cancel && (st.launches & 31) == 0) QM_HIP(hipStreamSynchronize(c->stream));
+  }
+  return st;
+}
+
+static State *as_state(qm_state *h) {
+  if (h == nullptr) fail(QM_ERR_ARG, "state is null");
+  State *s = reinterpret_cast<State *>(h);
+  use_device(s->ctx);
+  return s;
+}
+
+static void noise_apply(State *s, int kind, double p, int target, const double *u, int64_t n_u,
+                        int64_t *cursor);
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+const char *qm_last_error(void) { return g_error.c_str(); }
+int32_t qm_abi_version(void) { return QM_ABI_VERSION; }
+
+int qm_device_count(int32_t *count) {
+  return guarded([&] {
+    if (!count) fail(QM_ERR_ARG, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int qm_device_get_info(int32_t index, qm_device_info *out) {
+  return guarded([&] {
+    if (!out) fail(QM_ERR_ARG, "info is null");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+      fail(QM_ERR_NODEVICE, "No compatible AMD HIP device/driver was found.");
+    if (index < 0 || index >= n) fail(QM_ERR_RANGE, "HIP device index is not available.");
+    hipDeviceProp_t p;
+    QM_HIP(hipGetDeviceProperties(&p, index));
+    std::memset(out, 0, sizeof(*out));
+    out->index = index;
+    out->compute_units = p.multiProcessorCount;
+    out->memory_bytes = (int64_t)p.totalGlobalMem;
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    int drv = 0;
+    (void)hipDriverGetVersion(&drv);
+    std::snprintf(out->driver, sizeof(out->driver), "%d", drv);
+  });
+}
+
+int qm_ctx_create(const qm_ctx_options *opt, qm_ctx **out) {
+  return guarded([&] {
+    if (!out) fail(QM_ERR_ARG, "out is null");
+    *out = nullptr;
+    qm_ctx_options o{};
+    if (opt) o = *opt;
+    // CudaBackendOptions.Validate (CudaBackendOptions.cs:14-19); 0 = no budget.
+    if (o.device_index < 0) fail(QM_ERR_RANGE, "DeviceIndex must be nonnegative.");
+    if (o.device_budget_bytes != 0 && o.device_budget_bytes < 32)
+      fail(QM_ERR_RANGE, "DeviceMemoryBudgetBytes must be at least 32.");
+    if (o.staging_budget_bytes != 0 && o.staging_budget_bytes < 32)
+      fail(QM_ERR_RANGE, "StagingMemoryBudgetBytes must be at least 32.");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+      fail(QM_ERR_NODEVICE,
+           "No compatible AMD HIP device/driver was found. Select a CPU backend explicitly.");
+    if (o.device_index >= n) fail(QM_ERR_RANGE, "HIP device index is not available.");
+    Ctx *c = new Ctx();
+    try {
+      c->device = o.device_index;
+      c->profiling = o.enable_profiling != 0;
+      c->device_budget = o.device_budget_bytes;
+      c->staging_budget = o.staging_budget_bytes;
+      QM_HIP(hipSetDevice(c->device));
+      hipDeviceProp_t p;
+      QM_HIP(hipGetDeviceProperties(&p, c->device));
+      c->device_memory = (int64_t)p.totalGlobalMem;
+      c->compute_units = p.multiProcessorCount;
+      QM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      QM_HIP(hipEventCreate(&c->ev_start));
+      QM_HIP(hipEventCreate(&c->ev_stop));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = reinterpret_cast<qm_ctx *>(c);
+  });
+}
+
+int qm_ctx_destroy(qm_ctx *h) {
+  return guarded([&] {
+    if (!h) return;
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    if (c->disposed) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->d_partial) (void)hipFree(c->d_partial);
+    if (c->d_ipartial) (void)hipFree(c->d_ipartial);
+    if (c->d_prog) (void)hipFree(c->d_prog);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->disposed = true;
+    // The Ctx object itself stays allocated so late calls observe
+    // QM_ERR_DISPOSED (CudaStateVectorBackend.cs:212-214).
+  });
+}
+
+int qm_ctx_resident_bytes(qm_ctx *h, int64_t *bytes) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    use_device(c);
+    if (!bytes) fail(QM_ERR_ARG, "bytes is null");
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    *bytes = c->resident_bytes;
+  });
+}
+
+static State *state_new(Ctx *c, int qubits, int precision) {
+  if (qubits < 1 || qubits > 34) fail(QM_ERR_RANGE, "qubits must be between 1 and 34.");
+  if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+  const int64_t bytes = amp_bytes(precision) << qubits;
+  if ((c->device_budget && c->resident_bytes + bytes > c->device_budget) ||
+      c->resident_bytes + bytes > c->device_memory)
+    fail(QM_ERR_BUDGET, "HIP state exceeds the device memory budget.");
+  State *s = new State();
+  s->ctx = c;
+  s->qubits = qubits;
+  s->precision = precision;
+  s->length = (int64_t)1 << qubits;
+  s->bytes = bytes;
+  hipError_t e = hipMalloc(&s->data, (size_t)bytes);
+  if (e != hipSuccess) {
+    delete s;
+    (void)hipGetLastError();
+    fail(QM_ERR_BUDGET, "HIP state exceeds the free device memory.");
+  }
+  c->resident_bytes += bytes;
+  return s;
+}
+
+static void state_free(State *s) {
+  Ctx *c = s->ctx;
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(s->data);
+  c->resident_bytes -= s->bytes;
+  delete s;
+}
+
+int qm_execute(qm_ctx *h, int32_t qubits, const qm_op *ops, int64_t n_ops, double *amps,
+               int64_t n_amps, const volatile int32_t *cancel, qm_profile *profile) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    if (!c) fail(QM_ERR_ARG, "context is null");
+    if (profile && !c->profiling)
+      fail(QM_ERR_STATE, "EnableProfiling must be selected when creating the HIP backend.");
+    if (qubits < 1 || qubits > 34) fail(QM_ERR_RANGE, "qubits must be between 1 and 34.");
+    if (!amps || n_amps != ((int64_t)1 << qubits))
+      fail(QM_ERR_ARG, "Buffer length must equal 2^qubitCount.");
+    check_cancel(cancel);
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    use_device(c);
+    check_cancel(cancel);
+    const int64_t bytes = 16 * n_amps;
+    if ((c->device_budget && bytes > c->device_budget) || bytes > c->device_memory ||
+        (c->staging_budget && bytes > c->staging_budget))
+      fail(QM_ERR_BUDGET, "HIP state exceeds the device or host staging memory budget.");
+    std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
+    State *s = state_new(c, qubits, QM_C128);
+    try {
+      QM_HIP(hipMemcpyAsync(s->data, amps, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+      if (profile) QM_HIP(hipEventRecord(c->ev_start, c->stream));
+      ExecStats es = run_gates(s, gates, QM_SCHED_FUSED, cancel);
+      if (profile) QM_HIP(hipEventRecord(c->ev_stop, c->stream));
+      QM_HIP(hipStreamSynchronize(c->stream));
+      check_cancel(cancel);
+      // Commit only after the device work and the cancellation check (:166-176).
+      QM_HIP(hipMemcpyAsync(amps, s->data, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+      QM_HIP(hipStreamSynchronize(c->stream));
+      if (profile) {
+        float ms = 0;
+        QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+        profile->device_ms = ms;
+        profile->state_bytes = bytes;
+        profile->kernel_launches = es.launches;
+        profile->passes = es.passes;
+        profile->algorithmic_bytes = es.bytes;
+      }
+    } catch (...) {
+      state_free(s);
+      throw;
+    }
+    state_free(s);
+  });
+}
+
+int qm_state_create(qm_ctx *h, int32_t qubits, int32_t precision, qm_state **out) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    use_device(c);
+    if (!out) fail(QM_ERR_ARG, "out is null");
+    *out = nullptr;
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    State *s = state_new(c, qubits, precision);
+    try {
+      launch_set_basis(s, 0, c->stream);
+      QM_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      state_free(s);
+      throw;
+    }
+    *out = reinterpret_cast<qm_state *>(s);
+  });
+}
+
+int qm_state_destroy(qm_state *h) {
+  return guarded([&] {
+    if (!h) return;
+    State *s = reinterpret_cast<State *>(h);
+    Ctx *c = s->ctx;
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    if (!c->disposed) (void)hipSetDevice(c->device);
+    state_free(s);
+  });
+}
+
+int qm_state_info(qm_state *h, int32_t *qubits, int32_t *precision) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (qubits) *qubits = s->qubits;
+    if (precision) *precision = s->precision;
+  });
+}
+
+int qm_state_reset(qm_state *h) {
+  return guarded([&] {
+    State *s = as_state(h);
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    launch_set_basis(s, 0, s->ctx->stream);
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int qm_state_upload(qm_state *h, const double *amps, int64_t n) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!amps || n != s->length) fail(QM_ERR_ARG, "Buffer length must equal 2^qubitCount.");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    Ctx *c = s->ctx;
+    if (s->precision == QM_C128) {
+      QM_HIP(hipMemcpyAsync(s->data, amps, (size_t)s->bytes, hipMemcpyHostToDevice, c->stream));
+    } else {
+      // Round through a temporary complex128 state on the device.
+      State *tmp = state_new(c, s->qubits, QM_C128);
+      try {
+        QM_HIP(hipMemcpyAsync(tmp->data, amps, (size_t)tmp->bytes, hipMemcpyHostToDevice, c->stream));
+        launch_convert(tmp, s, c->stream);
+        QM_HIP(hipStreamSynchronize(c->stream));
+      } catch (...) {
+        state_free(tmp);
+        throw;
+      }
+      state_free(tmp);
+    }
+    QM_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int qm_state_download(qm_state *h, double *amps, int64_t n) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!amps || n != s->length) fail(QM_ERR_ARG, "Buffer length must equal 2^qubitCount.");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    Ctx *c = s->ctx;
+    if (s->precision == QM_C128) {
+      QM_HIP(hipMemcpyAsync(amps, s->data, (size_t)s->bytes, hipMemcpyDeviceToHost, c->stream));
+    } else {
+      State *tmp = state_new(c, s->qubits, QM_C128);
+      try {
+        launch_convert(s, tmp, c->stream);
+        QM_HIP(hipMemcpyAsync(amps, tmp->data, (size_t)tmp->bytes, hipMemcpyDeviceToHost, c->stream));
+        QM_HIP(hipStreamSynchronize(c->stream));
+      } catch (...) {
+        state_free(tmp);
+        throw;
+      }
+      state_free(tmp);
+    }
+    QM_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int qm_state_copy(qm_state *dst, qm_state *src) {
+  return guarded([&] {
+    State *d = as_state(dst);
+    State *s = as_state(src);
+    if (d->ctx != s->ctx) fail(QM_ERR_ARG, "states belong to different contexts");
+    if (d->qubits != s->qubits) fail(QM_ERR_ARG, "state widths differ");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    launch_convert(s, d, s->ctx->stream);
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int qm_state_execute(qm_state *h, const qm_op *ops, int64_t n_ops, int32_t schedule,
+                     const volatile int32_t *cancel, qm_profile *profile) {
+  return guarded([&] {
+    State *s = as_state(h);
+    Ctx *c = s->ctx;
+    if (profile && !c->profiling)
+      fail(QM_ERR_STATE, "EnableProfiling must be selected when creating the HIP backend.");
+    if (schedule != QM_SCHED_PER_GATE && schedule != QM_SCHED_FUSED)
+      fail(QM_ERR_ARG, "unknown schedule");
+    check_cancel(cancel);
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    std::vector<DevGate> gates = prepare_all(ops, n_ops, s->qubits);
+    if (profile) QM_HIP(hipEventRecord(c->ev_start, c->stream));
+    ExecStats es = run_gates(s, gates, schedule, cancel);
+    if (profile) QM_HIP(hipEventRecord(c->ev_stop, c->stream));
+    QM_HIP(hipStreamSynchronize(c->stream));
+    if (profile) {
+      float ms = 0;
+      QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+      profile->device_ms = ms;
+      profile->state_bytes = s->bytes;
+      profile->kernel_launches = es.launches;
+      profile->passes = es.passes;
+      profile->algorithmic_bytes = es.bytes;
+    }
+    check_cancel(cancel);
+  });
+}
+
+int qm_state_mass(qm_state *h, double *mass) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!mass) fail(QM_ERR_ARG, "mass is null");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    *mass = reduce_mass(s, s->ctx->stream).total;
+  });
+}
+
+static void validate_mass(double m) {
+  if (!std::isfinite(m) || !(m > 0))
+    fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+}
+
+int qm_state_measure_all(qm_state *h, double u, int64_t *outcome) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!outcome) fail(QM_ERR_ARG, "outcome is null");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    int64_t out = -1;
+    s->boundary_fallbacks += cdf_search(s, &u, 1, &out, s->ctx->stream);
+    // Collapse to |outcome> with amplitude +1 (StateVector.cs:150-151).
+    launch_set_basis(s, out, s->ctx->stream);
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+    *outcome = out;
+  });
+}
+
+static uint64_t selection_mask(const State *s, const int32_t *sel, int k) {
+  // ValidateSelection (StateVector.Observations.cs:143-168).
+  if (!sel || k < 1 || k > s->qubits) fail(QM_ERR_ARG, "Select between one and QubitCount distinct qubits.");
+  uint64_t m = 0;
+  for (int i = 0; i < k; i++) {
+    if (sel[i] < 0 || sel[i] >= s->qubits) fail(QM_ERR_RANGE, "Selected qubits must be in range.");
+    uint64_t b = 1ull << sel[i];
+    if (m & b) fail(QM_ERR_ARG, "Selected qubits must be distinct.");
+    m |= b;
+  }
+  return m;
+}
+
+int qm_state_measure(qm_state *h, const int32_t *sel, int32_t k, double u, int64_t *outcome) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!outcome) fail(QM_ERR_ARG, "outcome is null");
+    const uint64_t mask = selection_mask(s, sel, k);
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    int64_t basis = -1;
+    s->boundary_fallbacks += cdf_search(s, &u, 1, &basis, s->ctx->stream);
+    const uint64_t pattern = (uint64_t)basis & mask;
+    const double branch = reduce_masked_mass(s, mask, pattern, s->ctx->stream);
+    validate_mass(branch);
+    const double norm = std::sqrt(branch);
+    launch_collapse_subset(s, mask, pattern, norm, s->ctx->stream);
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+    int64_t out = 0;
+    for (int b = 0; b < k; b++) out |= (((uint64_t)basis >> sel[b]) & 1) << b;
+    *outcome = out;
+  });
+}
+
+int qm_state_marginal(qm_state *h, const int32_t *sel, int32_t k, int64_t outcome, double *p) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!p) fail(QM_ERR_ARG, "probability is null");
+    const uint64_t mask = selection_mask(s, sel, k);
+    if (outcome < 0 || outcome >= ((int64_t)1 << k)) fail(QM_ERR_RANGE, "outcome");
+    uint64_t pattern = 0;
+    for (int b = 0; b < k; b++) pattern |= (((uint64_t)outcome >> b) & 1) << sel[b];
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    const double total = reduce_mass(s, s->ctx->stream).total;
+    const double selected = reduce_masked_mass(s, mask, pattern, s->ctx->stream);
+    validate_mass(total);
+    *p = selected / total;
+  });
+}
+
+int qm_state_sample(qm_state *h, const double *u, int64_t shots, int64_t *outcomes) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (shots < 1) fail(QM_ERR_RANGE, "shots must be positive");
+    if (!u || !outcomes) fail(QM_ERR_ARG, "buffers are null");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    s->boundary_fallbacks += cdf_search(s, u, shots, outcomes, s->ctx->stream);
+  });
+}
+
+int qm_state_expectations(qm_state *h, const qm_pauli *terms, int64_t n, double *values) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (n < 0 || (n > 0 && (!terms || !values))) fail(QM_ERR_ARG, "buffers are null");
+    const uint64_t all = s->qubits >= 64 ? ~0ull : ((1ull << s->qubits) - 1);
+    for (int64_t i = 0; i < n; i++)
+      if ((terms[i].flip_mask | terms[i].sign_mask) & ~all)
+        fail(QM_ERR_ARG, "Observable and state sizes must match.");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    const double total = reduce_mass(s, s->ctx->stream).total;
+    validate_mass(total);
+    std::vector<double> num((size_t)n);
+    reduce_pauli(s, terms, n, num.data(), s->ctx->stream);
+    for (int64_t i = 0; i < n; i++) {
+      if (!std::isfinite(num[i])) fail(QM_ERR_NONFINITE, "The observable accumulation is nonfinite.");
+      values[i] = num[i] / total;
+    }
+  });
+}
+
+int qm_state_energy(qm_state *h, const qm_pauli *terms, const double *coeffs, int64_t n,
+                    double *energy) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!energy || n < 0 || (n > 0 && (!terms || !coeffs))) fail(QM_ERR_ARG, "buffers are null");
+    const uint64_t all = s->qubits >= 64 ? ~0ull : ((1ull << s->qubits) - 1);
+    for (int64_t i = 0; i < n; i++) {
+      if ((terms[i].flip_mask | terms[i].sign_mask) & ~all)
+        fail(QM_ERR_ARG, "Hamiltonian and state sizes must match.");
+      if (!std::isfinite(coeffs[i])) fail(QM_ERR_RANGE, "coefficient");
+    }
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    const double total = reduce_mass(s, s->ctx->stream).total;
+    validate_mass(total);
+    std::vector<qm_pauli> used;
+    std::vector<int64_t> idx;
+    for (int64_t i = 0; i < n; i++)
+      if (coeffs[i] != 0) {
+        used.push_back(terms[i]);
+        idx.push_back(i);
+      }
+    std::vector<double> num(used.size());
+    reduce_pauli(s, used.data(), (int64_t)used.size(), num.data(), s->ctx->stream);
+    // Neumaier summation in term order (StateVector.Energy.cs:26-45).
+    double sum = 0, corr = 0;
+    for (size_t k = 0; k < used.size(); k++) {
+      if (!std::isfinite(num[k])) fail(QM_ERR_NONFINITE, "The observable accumulation is nonfinite.");
+      const double value = coeffs[idx[k]] * (num[k] / total);
+      const double next = sum + value;
+      corr += std::fabs(sum) >= std::fabs(value) ? (sum - next) + value : (value - next) + sum;
+      sum = next;
+      if (!std::isfinite(sum) || !std::isfinite(corr))
+        fail(QM_ERR_NONFINITE, "The energy accumulation is nonfinite.");
+    }
+    const double e = sum + corr;
+    if (!std::isfinite(e)) fail(QM_ERR_NONFINITE, "The energy is nonfinite.");
+    *energy = e;
+  });
+}
+
+int qm_state_inner(qm_state *a, qm_state *b, double *re, double *im) {
+  return guarded([&] {
+    State *x = as_state(a);
+    State *y = as_state(b);
+    if (x->ctx != y->ctx || x->qubits != y->qubits) fail(QM_ERR_ARG, "states differ in width or context");
+    if (!re || !im) fail(QM_ERR_ARG, "outputs are null");
+    std::lock_guard<std::recursive_mutex> g(x->ctx->lock);
+    reduce_inner(x, y, re, im, x->ctx->stream);
+  });
+}
+
+int qm_state_boundary_fallbacks(qm_state *h, int64_t *count) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!count) fail(QM_ERR_ARG, "count is null");
+    *count = s->boundary_fallbacks;
+  });
+}
+
+int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n_ops,
+                  int32_t *passes, int32_t *transposes, int64_t *gates) {
+  return guarded([&] {
+    if (qubits < 1 || qubits > 34) fail(QM_ERR_RANGE, "qubits must be between 1 and 34.");
+    if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+    std::vector<DevGate> g = prepare_all(ops, n_ops, qubits);
+    const FusedShape shape = fused_shape(precision);
+    int32_t np = 0, nt = 0;
+    if (qubits >= shape.K && g.size() > 1) {
+      FusedPlan plan = plan_fused(g, qubits, shape);
+      np = (int32_t)plan.passes.size();
+      for (const FusedPass &p : plan.passes) nt += p.transposes;
+    } else {
+      np = (int32_t)g.size();
+    }
+    if (passes) *passes = np;
+    if (transposes) *transposes = nt;
+    if (gates) *gates = (int64_t)g.size();
+  });
+}
+
+int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
+                       int32_t lane_bits, int32_t contiguous, double *amps) {
+  return guarded([&] {
+    if (qubits < 1 || qubits > 26) fail(QM_ERR_RANGE, "host reference is limited to 26 qubits");
+    if (!amps) fail(QM_ERR_ARG, "amplitudes are null");
+    std::vector<DevGate> g = prepare_all(ops, n_ops, qubits);
+    FusedShape shape{tile_bits, lane_bits, tile_bits - lane_bits, contiguous};
+    if (shape.R < 1 || shape.R > 5 || shape.L < 1 || shape.L > 10 || shape.K > kMaxK ||
+        contiguous < 0 || contiguous >= shape.K)
+      fail(QM_ERR_RANGE, "unsupported tile shape");
+    if (qubits < shape.K) {
+      for (const DevGate &x : g) apply_gate_host(x, amps, qubits);
+      return;
+    }
+    FusedPlan plan = plan_fused(g, qubits, shape);
+    apply_plan_host(plan, amps, qubits);
+  });
+}
+
+int qm_state_noise(qm_state *h, int32_t kind, double p, int32_t target, const double *u,
+                   int64_t n_u, int64_t *cursor) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!cursor) fail(QM_ERR_ARG, "cursor is null");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    noise_apply(s, kind, p, target, u, n_u, cursor);
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int qm_state_noisy_execute(qm_state *h, const qm_step *steps, int64_t n_steps, const qm_op *ops,
+                           const double *u, int64_t n_u, int64_t *cursor) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!cursor || n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "buffers are null");
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    // Consecutive unitary segments run as one fused sequence.
+    std::vector<DevGate> run;
+    auto flush = [&] {
+      if (!run.empty()) run_gates(s, run, QM_SCHED_FUSED, nullptr);
+      run.clear();
+    };
+    for (int64_t k = 0; k < n_steps; k++) {
+      const qm_step &st = steps[k];
+      if (!st.is_channel) {
+        if (st.op_begin < 0 || st.op_count < 0) fail(QM_ERR_ARG, "bad segment");
+        std::vector<DevGate> g2 = prepare_all(ops + st.op_begin, st.op_count, s->qubits);
+        run.insert(run.end(), g2.begin(), g2.end());
+      } else if (st.probability != 0) {
+        flush();
+        noise_apply(s, st.noise_kind, st.probability, st.target, u, n_u, cursor);
+      } else if (!std::isfinite(st.probability) || st.probability < 0 || st.probability > 1) {
+        fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
+      }
+    }
+    flush();
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+}  // extern "C"
+
+namespace qm {
+
+static double take(const double *u, int64_t n, int64_t *cursor) {
+  if (!u || *cursor >= n) fail(QM_ERR_POOL, "The uniform pool is exhausted.");
+  return u[(*cursor)++];
+}
+
+// CpuNoiseChannelBackend.Apply (CpuNoiseChannelBackend.cs:10-145) on a resident state.
+static void noise_apply(State *s, int kind, double p, int target, const double *u, int64_t n_u,
+                        int64_t *cursor) {
+  if (target < 0 || target >= s->qubits) fail(QM_ERR_RANGE, "target");
+  if (!std::isfinite(p) || p < 0 || p > 1)
+    fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
+  if (kind < 0 || kind > 3) fail(QM_ERR_ARG, "unknown noise kind");
+  if (p == 0) return;
+  hipStream_t st = s->ctx->stream;
+  if (kind == 3) {
+    double m0, m1;
+    reduce_bit_masses(s, target, &m0, &m1, st);
+    double jump = p * m1, stay = m0 + (1 - p) * m1, total = jump + stay;
+    if (!std::isfinite(total) || total <= 0)
+      fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+    bool do_jump;
+    if (!(jump > 0)) {
+      do_jump = false;
+    } else if (stay == 0) {
+      do_jump = true;
+    } else {
+      const double r = take(u, n_u, cursor);
+      const double ratio = jump / total;
+      if (std::fabs(r - ratio) <= 1e-12 * ratio) {
+        // Within the summation-order margin: redo the masses in the
+        // reference's Kahan order before deciding.
+        serial_bit_masses(s, target, &m0, &m1, st);
+        jump = p * m1, stay = m0 + (1 - p) * m1, total = jump + stay;
+        s->boundary_fallbacks++;
+      }
+      do_jump = r < jump / total;
+    }
+    const double branch = do_jump ? jump : stay;
+    const double scale = 1 / std::sqrt(branch);
+    launch_scale_pairs(s, target, do_jump, scale, std::sqrt(p) * scale, std::sqrt(1 - p) * scale, st);
+    return;
+  }
+  int pauli;
+  if (kind == 2) {
+    const double r = take(u, n_u, cursor);
+    const double quarter = p / 4;
+    pauli = r < quarter ? 1 : r < 2 * quarter ? 2 : r < 3 * quarter ? 3 : 0;
+  } else {
+    const bool apply = p == 1 || take(u, n_u, cursor) < p;
+    pauli = !apply ? 0 : kind == 0 ? 1 : 3;
+  }
+  if (pauli == 0) return;
+  DevGate g{};
+  g.target = target;
+  g.swap_other = -1;
+  if (pauli == 1) {
+    g.kind = G_X;
+  } else if (pauli == 2) {
+    g.kind = G_Y;  // (one.im, -one.re), (-zero.im, zero.re): CpuNoiseChannelBackend.cs:77-78
+  } else {
+    g.kind = G_DIAG1;  // second = -one
+    g.c[0] = 1, g.c[6] = -1, g.c[7] = 0;
+  }
+  launch_gate(s, g, st);
+}
+
+}  // namespace qm

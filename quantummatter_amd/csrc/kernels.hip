@@ -1,0 +1,1000 @@
+// HIP kernels of libqmhip.so for gfx950 (CDNA4, 64-wide wavefronts).
+//
+// State layout: 2^n amplitudes, interleaved (re, im), 16 B (complex128) or
+// 8 B (complex64) each, qubit 0 the least significant index bit -- the
+// layout of System.Numerics.Complex[] in the reference (StateVector.cs:11).
+// All index arithmetic is 64-bit.  Built with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gate_math.hpp"
+#include "qm_internal.hpp"
+
+namespace qm {
+
+struct alignas(16) C128 {
+  double re, im;
+};
+struct alignas(8) C64 {
+  float re, im;
+};
+template <typename T> struct Real;
+template <> struct Real<C128> { using type = double; };
+template <> struct Real<C64> { using type = float; };
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Per-gate kernels: one launch per gate, enumerating only the amplitudes the
+// gate can change (pairs with controls satisfied; the one-half of a
+// diag(1, D)).  This is the reference's CUDA kernel shape
+// (CudaStateVectorKernel.cs:38-77) with 64-bit indices and touched-fraction
+// enumeration.
+// ---------------------------------------------------------------------------
+struct Fixed {
+  int32_t count;
+  int32_t pos[4];   // ascending bit positions inserted as zeros
+  uint64_t value;   // bits set after insertion
+};
+
+__device__ __forceinline__ uint64_t insert_bits(uint64_t i, const Fixed &f) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (k < f.count) {
+      const int p = f.pos[k];
+      i = ((i >> p) << (p + 1)) | (i & ((1ull << p) - 1));
+    }
+  }
+  return i | f.value;
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(kBlock) void k_gate(T *__restrict__ s, uint64_t items, Fixed f,
+                                                 uint64_t tbit, uint64_t obit, DevGate g) {
+  using F = typename Real<T>::type;
+  F c[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) c[i] = (F)g.c[i];
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < items; i += stride) {
+    const uint64_t first = insert_bits(i, f);
+    if constexpr (KIND == G_DIAG) {
+      T a = s[first];
+      const bool hi = (first & tbit) != 0;
+      mul_phase<F>(hi ? c[6] : c[0], hi ? c[7] : c[1], a.re, a.im);
+      s[first] = a;
+    } else if constexpr (KIND == G_DIAG1) {
+      T a = s[first];  // first has the target bit set (inserted as a fixed one)
+      mul_phase<F>(c[6], c[7], a.re, a.im);
+      s[first] = a;
+    } else {
+      const uint64_t second = KIND == G_SWAP ? (first ^ tbit ^ obit) : (first | tbit);
+      T x = s[first], y = s[second];
+      if constexpr (KIND == G_X || KIND == G_SWAP) {
+        s[first] = y;
+        s[second] = x;
+      } else {
+        if constexpr (KIND == G_MAT) pair_mat<F>(c, x.re, x.im, y.re, y.im);
+        if constexpr (KIND == G_RX) pair_rx<F>(c[0], c[1], x.re, x.im, y.re, y.im);
+        if constexpr (KIND == G_RY) pair_ry<F>(c[0], c[1], x.re, x.im, y.re, y.im);
+        if constexpr (KIND == G_H) pair_h<F>(c[0], x.re, x.im, y.re, y.im);
+        if constexpr (KIND == G_Y) pair_y<F>(x.re, x.im, y.re, y.im);
+        s[first] = x;
+        s[second] = y;
+      }
+    }
+  }
+}
+
+static int launch_blocks(uint64_t items) {
+  uint64_t b = (items + kBlock - 1) / kBlock;
+  return (int)std::min<uint64_t>(std::max<uint64_t>(b, 1), 65536);
+}
+
+template <typename T>
+static void launch_gate_t(T *s, int n, const DevGate &g, hipStream_t st) {
+  Fixed f{};
+  uint64_t tbit = 1ull << g.target, obit = 0;
+  std::vector<int> pos;
+  uint64_t value = g.ctrl_mask;
+  for (int q = 0; q < n; q++)
+    if (g.ctrl_mask & (1ull << q)) pos.push_back(q);
+  if (g.kind == G_SWAP) {
+    obit = 1ull << g.swap_other;
+    pos.push_back(g.swap_other);
+    value |= obit;  // a = 1, b = 0
+  }
+  if (g.kind != G_DIAG) pos.push_back(g.target);
+  if (g.kind == G_DIAG1) value |= tbit;
+  std::sort(pos.begin(), pos.end());
+  f.count = (int)pos.size();
+  for (int i = 0; i < f.count; i++) f.pos[i] = pos[i];
+  f.value = value;
+  const uint64_t items = 1ull << (n - f.count);
+  const int blocks = launch_blocks(items);
+  switch (g.kind) {
+#define QM_CASE(K) \
+  case K: k_gate<T, K><<<blocks, kBlock, 0, st>>>(s, items, f, tbit, obit, g); break;
+    QM_CASE(G_MAT)
+    QM_CASE(G_RX)
+    QM_CASE(G_RY)
+    QM_CASE(G_H)
+    QM_CASE(G_Y)
+    QM_CASE(G_X)
+    QM_CASE(G_DIAG)
+    QM_CASE(G_DIAG1)
+    QM_CASE(G_SWAP)
+#undef QM_CASE
+    default:
+      fail(QM_ERR_UNSUPPORTED, "unknown device gate kind");
+  }
+  QM_HIP(hipGetLastError());
+}
+
+void launch_gate(State *s, const DevGate &g, hipStream_t st) {
+  if (s->precision == QM_C128)
+    launch_gate_t<C128>((C128 *)s->data, s->qubits, g, st);
+  else
+    launch_gate_t<C64>((C64 *)s->data, s->qubits, g, st);
+}
+
+// ---------------------------------------------------------------------------
+// Fused tile pass.  One workgroup owns one tile of 2^K amplitudes at a time
+// (grid-stride over tiles); each thread keeps 2^R amplitudes in registers.
+// Gates on register bits run in registers; a new register set is reached by
+// an LDS transpose.  LDS is addressed through a 3-bit XOR swizzle so that 8
+// consecutive lanes touching power-of-two strides hit distinct 16-byte bank
+// groups.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t swz(uint32_t j) {
+  uint32_t h = j >> 3;
+  h ^= h >> 3;
+  h ^= h >> 6;
+  h ^= h >> 9;
+  return j ^ (h & 7);
+}
+
+template <typename F, int R, int B, int KIND>
+__device__ __forceinline__ void tile_pair(F *vr, F *vi, const double *cd, uint32_t reg_cmask) {
+  F c[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) c[i] = (F)cd[i];
+#pragma unroll
+  for (int r = 0; r < (1 << R); r++) {
+    if (r & (1 << B)) continue;
+    if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
+    const int r1 = r | (1 << B);
+    if constexpr (KIND == G_MAT) pair_mat<F>(c, vr[r], vi[r], vr[r1], vi[r1]);
+    if constexpr (KIND == G_RX) pair_rx<F>(c[0], c[1], vr[r], vi[r], vr[r1], vi[r1]);
+    if constexpr (KIND == G_RY) pair_ry<F>(c[0], c[1], vr[r], vi[r], vr[r1], vi[r1]);
+    if constexpr (KIND == G_H) pair_h<F>(c[0], vr[r], vi[r], vr[r1], vi[r1]);
+    if constexpr (KIND == G_Y) pair_y<F>(vr[r], vi[r], vr[r1], vi[r1]);
+    if constexpr (KIND == G_X) {
+      F tr = vr[r], ti = vi[r];
+      vr[r] = vr[r1], vi[r] = vi[r1];
+      vr[r1] = tr, vi[r1] = ti;
+    }
+  }
+}
+
+template <typename F, int R, int KIND>
+__device__ __forceinline__ void tile_pair_bit(int b, F *vr, F *vi, const double *c, uint32_t m) {
+  switch (b) {
+    case 0: tile_pair<F, R, 0, KIND>(vr, vi, c, m); break;
+    case 1: tile_pair<F, R, 1, KIND>(vr, vi, c, m); break;
+    case 2: tile_pair<F, R, 2, KIND>(vr, vi, c, m); break;
+    case 3: tile_pair<F, R, 3, KIND>(vr, vi, c, m); break;
+    default:
+      if constexpr (R > 4) tile_pair<F, R, 4, KIND>(vr, vi, c, m);
+      break;
+  }
+}
+
+template <typename F, int R>
+__device__ __forceinline__ void tile_op(const TileOp &op, F *vr, F *vi, uint32_t t, uint64_t base) {
+  if ((base & op.out_cmask) != op.out_cmask) return;
+  if ((t & op.lane_cmask) != op.lane_cmask) return;
+  if (op.rbit >= 0) {
+    switch (op.kind) {
+      case G_MAT: tile_pair_bit<F, R, G_MAT>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
+      case G_RX: tile_pair_bit<F, R, G_RX>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
+      case G_RY: tile_pair_bit<F, R, G_RY>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
+      case G_H: tile_pair_bit<F, R, G_H>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
+      case G_Y: tile_pair_bit<F, R, G_Y>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
+      case G_X: tile_pair_bit<F, R, G_X>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
+      default: break;
+    }
+    return;
+  }
+  // Diagonal: phase D where the target bit is one, A (or nothing for
+  // diag(1, D)) where it is zero.
+  const F ar = (F)op.c[0], ai = (F)op.c[1], dr = (F)op.c[6], di = (F)op.c[7];
+  const bool hi_out = (base & op.dout) != 0;
+  const bool hi_lane = (t & op.dlane) != 0;
+  const bool full = op.kind == G_DIAG;
+#pragma unroll
+  for (int r = 0; r < (1 << R); r++) {
+    if (((uint32_t)r & op.reg_cmask) != op.reg_cmask) continue;
+    const bool hi = hi_out || hi_lane || (((uint32_t)r & op.dreg) != 0);
+    if (full) {
+      mul_phase<F>(hi ? dr : ar, hi ? di : ai, vr[r], vi[r]);
+    } else if (hi) {
+      mul_phase<F>(dr, di, vr[r], vi[r]);
+    }
+  }
+}
+
+template <typename T, int L, int R>
+__global__ __launch_bounds__(1 << L) void k_fused(T *__restrict__ s, const DevPass *__restrict__ pass,
+                                                  const TileOp *__restrict__ prog, uint64_t tiles) {
+  using F = typename Real<T>::type;
+  constexpr int REGS = 1 << R;
+  extern __shared__ char lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
+  const DevPass &d = *pass;
+  const uint32_t t = threadIdx.x;
+  uint64_t load_lane = 0, store_lane = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    if ((t >> k) & 1) {
+      load_lane |= 1ull << d.load_lane_gbit[k];
+      store_lane |= 1ull << d.store_lane_gbit[k];
+    }
+  }
+  const int nph = d.nphases;
+  const TileOp *ops = prog + d.op_offset;
+  F vr[REGS], vi[REGS];
+
+  auto lane_local = [&](int slot) {
+    uint32_t j = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++)
+      if ((t >> k) & 1) j |= 1u << d.lanepos[slot][k];
+    return j;
+  };
+  auto transpose = [&](int from, int to) {
+    __syncthreads();
+    const uint32_t jf = lane_local(from);
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      T a;
+      a.re = vr[r];
+      a.im = vi[r];
+      lds[swz(jf | d.regloc[from][r])] = a;
+    }
+    __syncthreads();
+    const uint32_t jt = lane_local(to);
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      T a = lds[swz(jt | d.regloc[to][r])];
+      vr[r] = a.re;
+      vi[r] = a.im;
+    }
+  };
+
+  for (uint64_t tau = blockIdx.x; tau < tiles; tau += gridDim.x) {
+    uint64_t base = 0;
+    for (int k = 0; k < d.nruns; k++)
+      base |= ((tau >> d.run_src[k]) & ((1ull << d.run_len[k]) - 1)) << d.run_dst[k];
+    const uint64_t lb = base | load_lane;
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      T a = s[lb | d.load_reg_off[r]];
+      vr[r] = a.re;
+      vi[r] = a.im;
+    }
+    int cur = d.load_direct ? 0 : kIo;
+    for (int ph = 0; ph < nph; ph++) {
+      if (cur != ph) transpose(cur, ph);
+      cur = ph;
+      const TileOp *po = ops + d.phase_op_begin[ph];
+      const int cnt = d.phase_op_count[ph];
+      for (int o = 0; o < cnt; o++) tile_op<F, R>(po[o], vr, vi, t, base);
+    }
+    if (!d.store_direct) transpose(cur, kIo);
+    const uint64_t sb = base | store_lane;
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      T a;
+      a.re = vr[r];
+      a.im = vi[r];
+      s[sb | d.store_reg_off[r]] = a;
+    }
+  }
+}
+
+template <typename T, int L, int R>
+static void launch_fused_t(State *st, const FusedPass &p, const void *d_pass, const void *d_ops,
+                           hipStream_t stream) {
+  const int K = L + R;
+  const uint64_t tiles = 1ull << (st->qubits - K);
+  const size_t lds = sizeof(T) << K;
+  static bool configured = false;
+  if (!configured) {
+    QM_HIP(hipFuncSetAttribute((const void *)k_fused<T, L, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    configured = true;
+  }
+  const uint64_t want = (uint64_t)st->ctx->compute_units * 8;
+  const int blocks = (int)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
+  k_fused<T, L, R><<<blocks, 1 << L, lds, stream>>>((T *)st->data, (const DevPass *)d_pass,
+                                                     (const TileOp *)d_ops, tiles);
+  QM_HIP(hipGetLastError());
+}
+
+FusedShape fused_shape(int precision) {
+  // complex128: 4096-amplitude tiles (64 KiB LDS), 256 threads x 16 registers.
+  // complex64: 8192-amplitude tiles (64 KiB LDS), 256 threads x 32 registers.
+  if (precision == QM_C128) return FusedShape{12, 8, 4, 3};
+  return FusedShape{13, 8, 5, 4};
+}
+
+void launch_fused_pass(State *s, const FusedPass &p, const void *d_pass, const void *d_ops,
+                       hipStream_t st) {
+  if (s->precision == QM_C128)
+    launch_fused_t<C128, 8, 4>(s, p, d_pass, d_ops, st);
+  else
+    launch_fused_t<C64, 8, 5>(s, p, d_pass, d_ops, st);
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_basis(T *__restrict__ s, uint64_t n, uint64_t index) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    T a;
+    a.re = i == index ? 1 : 0;
+    a.im = 0;
+    s[i] = a;
+  }
+}
+
+void launch_set_basis(State *s, int64_t index, hipStream_t st) {
+  const uint64_t n = (uint64_t)s->length;
+  if (s->precision == QM_C128)
+    k_fill_basis<C128><<<launch_blocks(n), kBlock, 0, st>>>((C128 *)s->data, n, (uint64_t)index);
+  else
+    k_fill_basis<C64><<<launch_blocks(n), kBlock, 0, st>>>((C64 *)s->data, n, (uint64_t)index);
+  QM_HIP(hipGetLastError());
+}
+
+// ApplyDamping commit (CpuNoiseChannelBackend.cs:118-142).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_damp(T *__restrict__ s, uint64_t pairs, uint64_t bit,
+                                                 int jump, double scale, double jump_scale,
+                                                 double stay_scale) {
+  using F = typename Real<T>::type;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < pairs; p += stride) {
+    const uint64_t low = p & (bit - 1);
+    const uint64_t first = ((p - low) << 1) | low;
+    const uint64_t second = first | bit;
+    if (jump) {
+      T y = s[second];
+      T a;
+      a.re = (F)(jump_scale * (double)y.re);
+      a.im = (F)(jump_scale * (double)y.im);
+      s[first] = a;
+      T z;
+      z.re = 0;
+      z.im = 0;
+      s[second] = z;
+    } else {
+      T x = s[first], y = s[second];
+      x.re = (F)((double)x.re * scale);
+      x.im = (F)((double)x.im * scale);
+      y.re = (F)((double)y.re * stay_scale);
+      y.im = (F)((double)y.im * stay_scale);
+      s[first] = x;
+      s[second] = y;
+    }
+  }
+}
+
+void launch_scale_pairs(State *s, int target, bool jump, double scale, double jump_scale,
+                        double stay_scale, hipStream_t st) {
+  const uint64_t pairs = (uint64_t)s->length / 2, bit = 1ull << target;
+  if (s->precision == QM_C128)
+    k_damp<C128><<<launch_blocks(pairs), kBlock, 0, st>>>((C128 *)s->data, pairs, bit, jump, scale,
+                                                          jump_scale, stay_scale);
+  else
+    k_damp<C64><<<launch_blocks(pairs), kBlock, 0, st>>>((C64 *)s->data, pairs, bit, jump, scale,
+                                                         jump_scale, stay_scale);
+  QM_HIP(hipGetLastError());
+}
+
+// Measure commit (StateVector.Observations.cs:241-247): keep the branch divided
+// component-wise by the norm, zero the rest.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_collapse(T *__restrict__ s, uint64_t n, uint64_t mask,
+                                                     uint64_t pattern, double norm) {
+  using F = typename Real<T>::type;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    T a = s[i];
+    if ((i & mask) == pattern) {
+      a.re = (F)((double)a.re / norm);
+      a.im = (F)((double)a.im / norm);
+    } else {
+      a.re = 0;
+      a.im = 0;
+    }
+    s[i] = a;
+  }
+}
+
+void launch_collapse_subset(State *s, uint64_t mask, uint64_t pattern, double norm, hipStream_t st) {
+  const uint64_t n = (uint64_t)s->length;
+  if (s->precision == QM_C128)
+    k_collapse<C128><<<launch_blocks(n), kBlock, 0, st>>>((C128 *)s->data, n, mask, pattern, norm);
+  else
+    k_collapse<C64><<<launch_blocks(n), kBlock, 0, st>>>((C64 *)s->data, n, mask, pattern, norm);
+  QM_HIP(hipGetLastError());
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(kBlock) void k_convert(const A *__restrict__ a, B *__restrict__ b, uint64_t n) {
+  using F = typename Real<B>::type;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    A x = a[i];
+    B y;
+    y.re = (F)x.re;
+    y.im = (F)x.im;
+    b[i] = y;
+  }
+}
+
+void launch_convert(const State *src, State *dst, hipStream_t st) {
+  const uint64_t n = (uint64_t)src->length;
+  const int blocks = launch_blocks(n);
+  if (src->precision == QM_C128 && dst->precision == QM_C64)
+    k_convert<C128, C64><<<blocks, kBlock, 0, st>>>((const C128 *)src->data, (C64 *)dst->data, n);
+  else if (src->precision == QM_C64 && dst->precision == QM_C128)
+    k_convert<C64, C128><<<blocks, kBlock, 0, st>>>((const C64 *)src->data, (C128 *)dst->data, n);
+  else
+    QM_HIP(hipMemcpyAsync(dst->data, src->data, (size_t)src->bytes, hipMemcpyDeviceToDevice, st));
+  QM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions.  Block partials in a fixed order, then one block folds the
+// partials in a fixed order: results are deterministic run to run.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double prob_of(const T &a) {
+  const double re = (double)a.re, im = (double)a.im;
+  return re * re + im * im;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
+  return r;  // valid in thread 0
+}
+
+__device__ __forceinline__ long long block_max(long long v, long long *sh) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    long long u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  long long r = -1;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = sh[i] > r ? sh[i] : r;
+  return r;
+}
+
+// mode 0: total mass + last supported index; mode 1: masked mass; mode 2:
+// masses of bit 0 / bit 1 of `mask` (stored as partial pairs).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_mass(const T *__restrict__ s, uint64_t n, uint64_t per,
+                                                 uint64_t mask, uint64_t pattern, double *partial,
+                                                 long long *ipartial) {
+  __shared__ double sh[kBlock / 64];
+  __shared__ long long shl[kBlock / 64];
+  const uint64_t begin = (uint64_t)blockIdx.x * per;
+  const uint64_t end = begin + per < n ? begin + per : n;
+  double a0 = 0, a1 = 0;
+  long long last = -1;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kBlock) {
+    const double p = prob_of(s[i]);
+    if constexpr (MODE == 0) {
+      a0 += p;
+      if (p > 0) last = (long long)i;
+    } else if constexpr (MODE == 1) {
+      if ((i & mask) == pattern) a0 += p;
+    } else {
+      if (i & mask)
+        a1 += p;
+      else
+        a0 += p;
+    }
+  }
+  const double r0 = block_sum(a0, sh);
+  if constexpr (MODE == 2) {
+    const double r1 = block_sum(a1, sh);
+    if (threadIdx.x == 0) {
+      partial[2 * blockIdx.x] = r0;
+      partial[2 * blockIdx.x + 1] = r1;
+    }
+  } else {
+    if (threadIdx.x == 0) partial[blockIdx.x] = r0;
+  }
+  if constexpr (MODE == 0) {
+    const long long m = block_max(last, shl);
+    if (threadIdx.x == 0) ipartial[blockIdx.x] = m;
+  }
+}
+
+// Folds partial columns: out[c] = sum_b partial[b*bstride + c*cstride], b < count.
+__global__ __launch_bounds__(kBlock) void k_fold(const double *__restrict__ partial, int64_t count,
+                                                 int64_t bstride, int64_t cstride, double *out,
+                                                 const long long *__restrict__ ipartial,
+                                                 long long *iout) {
+  __shared__ double sh[kBlock / 64];
+  __shared__ long long shl[kBlock / 64];
+  const int64_t c = blockIdx.x;
+  double a = 0;
+  for (int64_t b = threadIdx.x; b < count; b += kBlock) a += partial[b * bstride + c * cstride];
+  const double r = block_sum(a, sh);
+  if (threadIdx.x == 0) out[c] = r;
+  if (ipartial) {
+    long long m = -1;
+    for (int64_t b = threadIdx.x; b < count; b += kBlock) m = ipartial[b] > m ? ipartial[b] : m;
+    const long long mm = block_max(m, shl);
+    if (threadIdx.x == 0) *iout = mm;
+  }
+}
+
+static int64_t reduction_blocks(uint64_t n, uint64_t *per) {
+  uint64_t blocks = std::min<uint64_t>(std::max<uint64_t>((n + 4095) / 4096, 1), 4096);
+  *per = (n + blocks - 1) / blocks;
+  blocks = (n + *per - 1) / *per;
+  return (int64_t)blocks;
+}
+
+template <int MODE>
+static void run_mass(State *s, uint64_t mask, uint64_t pattern, double *out_host,
+                     long long *last_host, hipStream_t st) {
+  uint64_t per;
+  const int64_t blocks = reduction_blocks((uint64_t)s->length, &per);
+  const int width = MODE == 2 ? 2 : 1;
+  double *part = ctx_partials(s->ctx, blocks * width + 8);
+  int64_t *ipart = ctx_ipartials(s->ctx, blocks + 8);
+  double *res = part + blocks * width;  // width doubles of results
+  long long *ires = (long long *)(ipart + blocks);
+  if (s->precision == QM_C128)
+    k_mass<C128, MODE><<<(int)blocks, kBlock, 0, st>>>((const C128 *)s->data, (uint64_t)s->length, per,
+                                                       mask, pattern, part, (long long *)ipart);
+  else
+    k_mass<C64, MODE><<<(int)blocks, kBlock, 0, st>>>((const C64 *)s->data, (uint64_t)s->length, per,
+                                                      mask, pattern, part, (long long *)ipart);
+  QM_HIP(hipGetLastError());
+  k_fold<<<width, kBlock, 0, st>>>(part, blocks, width, 1, res,
+                                    MODE == 0 ? (long long *)ipart : nullptr, ires);
+  QM_HIP(hipGetLastError());
+  double *pin = ctx_pinned(s->ctx, 4);
+  QM_HIP(hipMemcpyAsync(pin, res, sizeof(double) * width, hipMemcpyDeviceToHost, st));
+  if (MODE == 0) QM_HIP(hipMemcpyAsync(pin + 2, ires, sizeof(long long), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < width; i++) out_host[i] = pin[i];
+  if (MODE == 0) std::memcpy(last_host, pin + 2, sizeof(long long));
+}
+
+MassResult reduce_mass(State *s, hipStream_t st) {
+  double total;
+  long long last;
+  run_mass<0>(s, 0, 0, &total, &last, st);
+  return MassResult{total, (int64_t)last};
+}
+
+double reduce_masked_mass(State *s, uint64_t mask, uint64_t pattern, hipStream_t st) {
+  double v;
+  run_mass<1>(s, mask, pattern, &v, nullptr, st);
+  return v;
+}
+
+void reduce_bit_masses(State *s, int target, double *m0, double *m1, hipStream_t st) {
+  double v[2];
+  run_mass<2>(s, 1ull << target, 0, v, nullptr, st);
+  *m0 = v[0];
+  *m1 = v[1];
+}
+
+// Kahan masses in the reference's sequential order (CpuNoiseChannelBackend.cs:88-106).
+template <typename T>
+__global__ void k_serial_bit_masses(const T *__restrict__ s, uint64_t n, uint64_t bit, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double z = 0, o = 0, zc = 0, oc = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    const double m = prob_of(s[i]);
+    double &sum = (i & bit) ? o : z;
+    double &comp = (i & bit) ? oc : zc;
+    const double corrected = m - comp;
+    const double next = sum + corrected;
+    comp = (next - sum) - corrected;
+    sum = next;
+  }
+  out[0] = z;
+  out[1] = o;
+}
+
+void serial_bit_masses(State *s, int target, double *m0, double *m1, hipStream_t st) {
+  double *res = ctx_partials(s->ctx, 2);
+  if (s->precision == QM_C128)
+    k_serial_bit_masses<C128><<<1, 64, 0, st>>>((const C128 *)s->data, (uint64_t)s->length, 1ull << target, res);
+  else
+    k_serial_bit_masses<C64><<<1, 64, 0, st>>>((const C64 *)s->data, (uint64_t)s->length, 1ull << target, res);
+  QM_HIP(hipGetLastError());
+  double *pin = ctx_pinned(s->ctx, 2);
+  QM_HIP(hipMemcpyAsync(pin, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  *m0 = pin[0];
+  *m1 = pin[1];
+}
+
+// Pauli numerators (StateVector.Observations.cs:269-286): for each term of a
+// flip-mask group, sum_j (-1)^popcount(j & sign) f_y(conj(a[j ^ flip]) * a[j]).
+constexpr int kTermsPerGroup = 8;
+struct PauliGroup {
+  uint64_t flip;
+  int32_t count;
+  int32_t first;  // index of the group's first term in the output order
+  uint64_t sign[kTermsPerGroup];
+  int32_t y[kTermsPerGroup];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pauli(const T *__restrict__ s, uint64_t n, uint64_t per,
+                                                  const PauliGroup *__restrict__ groups,
+                                                  double *partial, int64_t blocks) {
+  __shared__ double sh[kBlock / 64];
+  const PauliGroup &g = groups[blockIdx.y];
+  const uint64_t begin = (uint64_t)blockIdx.x * per;
+  const uint64_t end = begin + per < n ? begin + per : n;
+  double acc[kTermsPerGroup];
+#pragma unroll
+  for (int k = 0; k < kTermsPerGroup; k++) acc[k] = 0;
+  for (uint64_t j = begin + threadIdx.x; j < end; j += kBlock) {
+    const T a = s[j];
+    const T p = s[j ^ g.flip];
+    const double ar = a.re, ai = a.im, pr = p.re, pim = -(double)p.im;
+    // Complex.Conjugate(partner) * a.
+    const double ore = pr * ar - pim * ai;
+    const double oim = pim * ar + pr * ai;
+#pragma unroll
+    for (int k = 0; k < kTermsPerGroup; k++) {
+      if (k < g.count) {
+        const int y = g.y[k] & 3;
+        double c = y == 0 ? ore : y == 1 ? -oim : y == 2 ? -ore : oim;
+        if (__popcll(j & g.sign[k]) & 1) c = -c;
+        acc[k] += c;
+      }
+    }
+  }
+  for (int k = 0; k < kTermsPerGroup; k++) {
+    if (k >= g.count) break;
+    const double r = block_sum(acc[k], sh);
+    if (threadIdx.x == 0) partial[(int64_t)(g.first + k) * blocks + blockIdx.x] = r;
+  }
+}
+
+void reduce_pauli(State *s, const qm_pauli *terms, int64_t n_terms, double *numerators,
+                  hipStream_t st) {
+  if (n_terms == 0) return;
+  // Group terms by flip mask; groups hold up to kTermsPerGroup terms.
+  std::vector<int64_t> order(n_terms);
+  for (int64_t i = 0; i < n_terms; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return terms[a].flip_mask < terms[b].flip_mask; });
+  std::vector<PauliGroup> groups;
+  for (int64_t k = 0; k < n_terms;) {
+    PauliGroup g{};
+    g.flip = terms[order[k]].flip_mask;
+    g.first = (int32_t)k;
+    while (k < n_terms && g.count < kTermsPerGroup && terms[order[k]].flip_mask == g.flip) {
+      g.sign[g.count] = terms[order[k]].sign_mask;
+      g.y[g.count] = terms[order[k]].y_count;
+      g.count++;
+      k++;
+    }
+    groups.push_back(g);
+  }
+  uint64_t per;
+  const int64_t blocks = reduction_blocks((uint64_t)s->length, &per);
+  const int64_t ng = (int64_t)groups.size();
+  const int64_t gbytes = ng * (int64_t)sizeof(PauliGroup);
+  double *part = ctx_partials(s->ctx, n_terms * blocks + n_terms + (gbytes + 7) / 8);
+  double *res = part + n_terms * blocks;
+  PauliGroup *d_groups = (PauliGroup *)(res + n_terms);
+  QM_HIP(hipMemcpyAsync(d_groups, groups.data(), (size_t)gbytes, hipMemcpyHostToDevice, st));
+  for (int64_t g0 = 0; g0 < ng; g0 += 65535) {
+    const int64_t gn = std::min<int64_t>(65535, ng - g0);
+    dim3 grid((unsigned)blocks, (unsigned)gn);
+    if (s->precision == QM_C128)
+      k_pauli<C128><<<grid, kBlock, 0, st>>>((const C128 *)s->data, (uint64_t)s->length, per,
+                                             d_groups + g0, part, blocks);
+    else
+      k_pauli<C64><<<grid, kBlock, 0, st>>>((const C64 *)s->data, (uint64_t)s->length, per,
+                                            d_groups + g0, part, blocks);
+    QM_HIP(hipGetLastError());
+  }
+  // Fold: for term k (in sorted order), the partial row k.
+  k_fold<<<(unsigned)n_terms, kBlock, 0, st>>>(part, blocks, 1, blocks, res, nullptr, nullptr);
+  QM_HIP(hipGetLastError());
+  std::vector<double> sorted(n_terms);
+  double *pin = ctx_pinned(s->ctx, n_terms);
+  QM_HIP(hipMemcpyAsync(pin, res, sizeof(double) * n_terms, hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  for (int64_t k = 0; k < n_terms; k++) numerators[order[k]] = pin[k];
+}
+
+// Inner product <a|b> with FP64 accumulation.
+template <typename A, typename B>
+__global__ __launch_bounds__(kBlock) void k_inner(const A *__restrict__ a, const B *__restrict__ b,
+                                                  uint64_t n, uint64_t per, double *partial) {
+  __shared__ double sh[kBlock / 64];
+  const uint64_t begin = (uint64_t)blockIdx.x * per;
+  const uint64_t end = begin + per < n ? begin + per : n;
+  double re = 0, im = 0;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kBlock) {
+    const double ar = a[i].re, ai = a[i].im, br = b[i].re, bi = b[i].im;
+    re += ar * br + ai * bi;
+    im += ar * bi - ai * br;
+  }
+  const double r0 = block_sum(re, sh);
+  const double r1 = block_sum(im, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = r0;
+    partial[2 * blockIdx.x + 1] = r1;
+  }
+}
+
+void reduce_inner(State *a, State *b, double *re, double *im, hipStream_t st) {
+  uint64_t per;
+  const int64_t blocks = reduction_blocks((uint64_t)a->length, &per);
+  double *part = ctx_partials(a->ctx, 2 * blocks + 2);
+  double *res = part + 2 * blocks;
+  const uint64_t n = (uint64_t)a->length;
+  if (a->precision == QM_C128 && b->precision == QM_C128)
+    k_inner<C128, C128><<<(int)blocks, kBlock, 0, st>>>((const C128 *)a->data, (const C128 *)b->data, n, per, part);
+  else if (a->precision == QM_C128)
+    k_inner<C128, C64><<<(int)blocks, kBlock, 0, st>>>((const C128 *)a->data, (const C64 *)b->data, n, per, part);
+  else if (b->precision == QM_C128)
+    k_inner<C64, C128><<<(int)blocks, kBlock, 0, st>>>((const C64 *)a->data, (const C128 *)b->data, n, per, part);
+  else
+    k_inner<C64, C64><<<(int)blocks, kBlock, 0, st>>>((const C64 *)a->data, (const C64 *)b->data, n, per, part);
+  QM_HIP(hipGetLastError());
+  k_fold<<<2, kBlock, 0, st>>>(part, blocks, 2, 1, res, nullptr, nullptr);
+  QM_HIP(hipGetLastError());
+  double *pin = ctx_pinned(a->ctx, 2);
+  QM_HIP(hipMemcpyAsync(pin, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  *re = pin[0];
+  *im = pin[1];
+}
+
+// ---------------------------------------------------------------------------
+// CDF search (StateVectorSimulator.cs:44-90, StateVector.cs:131-147).
+//
+// Chunk sums, an exclusive scan of the chunk sums, then one thread per draw
+// binary-searches the chunk prefix and walks its chunk sequentially.  Both
+// the reference's left-to-right sum and this hierarchical sum are within
+// gamma_N * total of the exact prefix (N terms, all non-negative), so a draw
+// farther than delta = (5N + 8) * 2^-53 * total from both adjacent
+// cumulative values gets the reference's decision.  Draws inside that margin
+// are flagged and re-decided by a single-lane sequential pass in the
+// reference's order.
+// ---------------------------------------------------------------------------
+constexpr int64_t kChunk = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ s, uint64_t n,
+                                                       uint64_t chunk, double *sums) {
+  __shared__ double sh[kBlock / 64];
+  const uint64_t begin = (uint64_t)blockIdx.x * chunk;
+  const uint64_t end = begin + chunk < n ? begin + chunk : n;
+  double a = 0;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += kBlock) a += prob_of(s[i]);
+  const double r = block_sum(a, sh);
+  if (threadIdx.x == 0) sums[blockIdx.x] = r;
+}
+
+// Exclusive scan of `count` chunk sums by one 1024-thread block; total at out[count].
+__global__ __launch_bounds__(1024) void k_scan(const double *__restrict__ sums, int64_t count,
+                                               double *prefix) {
+  __shared__ double sh[1024];
+  const int t = threadIdx.x;
+  const int64_t seg = (count + 1023) / 1024;
+  const int64_t b = t * seg, e = b + seg < count ? b + seg : count;
+  double local = 0;
+  for (int64_t i = b; i < e; i++) local += sums[i];
+  sh[t] = local;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    double v = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  double run = t == 0 ? 0 : sh[t - 1];
+  for (int64_t i = b; i < e; i++) {
+    prefix[i] = run;
+    run += sums[i];
+  }
+  if (t == 1023) prefix[count] = sh[1023];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_search(const T *__restrict__ s, uint64_t n, uint64_t chunk,
+                                                   const double *__restrict__ prefix, int64_t nchunks,
+                                                   const double *__restrict__ draws, int64_t shots,
+                                                   long long last, double delta, long long *out,
+                                                   int *ambiguous) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= shots) return;
+  const double draw = draws[k];
+  const double total = prefix[nchunks];
+  if (draw >= total) {  // only by rounding of u*total: always re-decided
+    out[k] = last;
+    ambiguous[k] = 1;
+    return;
+  }
+  // Last chunk whose prefix is <= draw.
+  int64_t lo = 0, hi = nchunks - 1;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo + 1) / 2;
+    if (prefix[mid] <= draw)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const uint64_t begin = (uint64_t)lo * chunk;
+  const uint64_t end = begin + chunk < n ? begin + chunk : n;
+  double cum = prefix[lo];
+  long long found = -1;
+  double before = cum, after = cum;
+  for (uint64_t i = begin; i < end; i++) {
+    const double p = prob_of(s[i]);
+    before = cum;
+    cum += p;
+    if (draw < cum) {
+      found = (long long)i;
+      after = cum;
+      break;
+    }
+  }
+  if (found < 0) {
+    out[k] = -1;
+    ambiguous[k] = 1;
+    return;
+  }
+  out[k] = found;
+  ambiguous[k] = (draw - before) <= delta || (after - draw) <= delta;
+}
+
+// Sequential resolution in the reference's order: total mass, then for each
+// flagged draw (ascending u) the first index whose running sum exceeds u*total.
+template <typename T>
+__global__ void k_serial_resolve(const T *__restrict__ s, uint64_t n, const double *__restrict__ u,
+                                 int64_t count, long long *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double total = 0;
+  long long last = -1;
+  for (uint64_t i = 0; i < n; i++) {
+    const double p = prob_of(s[i]);
+    total += p;
+    if (p > 0) last = (long long)i;
+  }
+  double cum = 0;
+  uint64_t i = 0;
+  for (int64_t k = 0; k < count; k++) {
+    const double draw = u[k] * total;
+    long long found = last;
+    while (i < n) {
+      const double next = cum + prob_of(s[i]);
+      if (draw < next) {
+        found = (long long)i;
+        break;
+      }
+      cum = next;
+      i++;
+    }
+    out[k] = found;
+  }
+}
+
+int64_t cdf_search(State *s, const double *uniforms, int64_t shots, int64_t *outcomes, hipStream_t st,
+                   double *total_out) {
+  const uint64_t n = (uint64_t)s->length;
+  const uint64_t chunk = std::min<uint64_t>((uint64_t)kChunk, n);
+  const int64_t nchunks = (int64_t)((n + chunk - 1) / chunk);
+  // Last supported index (and a first validation) before the scratch below is taken.
+  MassResult m = reduce_mass(s, st);
+  if (!std::isfinite(m.total) || !(m.total > 0))
+    fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+  // Scratch: sums[nchunks], prefix[nchunks+1], draws[shots], out[shots], amb[shots].
+  const int64_t need = nchunks + (nchunks + 1) + shots + shots + (shots + 1) / 2 + 4;
+  double *buf = ctx_partials(s->ctx, need);
+  double *sums = buf, *prefix = sums + nchunks, *draws = prefix + nchunks + 1;
+  long long *out = (long long *)(draws + shots);
+  int *amb = (int *)(out + shots);
+  if (s->precision == QM_C128)
+    k_chunk_sums<C128><<<(int)nchunks, kBlock, 0, st>>>((const C128 *)s->data, n, chunk, sums);
+  else
+    k_chunk_sums<C64><<<(int)nchunks, kBlock, 0, st>>>((const C64 *)s->data, n, chunk, sums);
+  QM_HIP(hipGetLastError());
+  k_scan<<<1, 1024, 0, st>>>(sums, nchunks, prefix);
+  QM_HIP(hipGetLastError());
+  double *pin = ctx_pinned(s->ctx, 1);
+  QM_HIP(hipMemcpyAsync(pin, prefix + nchunks, sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  const double total = pin[0];
+  if (!std::isfinite(total) || !(total > 0))
+    fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+  if (total_out) *total_out = total;
+  std::vector<double> d(shots);
+  for (int64_t k = 0; k < shots; k++) d[k] = uniforms[k] * total;
+  QM_HIP(hipMemcpyAsync(draws, d.data(), sizeof(double) * shots, hipMemcpyHostToDevice, st));
+  const double delta = (5.0 * (double)n + 8.0) * 0x1p-53 * total;
+  const int blocks = (int)((shots + kBlock - 1) / kBlock);
+  if (s->precision == QM_C128)
+    k_search<C128><<<blocks, kBlock, 0, st>>>((const C128 *)s->data, n, chunk, prefix, nchunks, draws,
+                                              shots, (long long)m.last_supported, delta, out, amb);
+  else
+    k_search<C64><<<blocks, kBlock, 0, st>>>((const C64 *)s->data, n, chunk, prefix, nchunks, draws,
+                                             shots, (long long)m.last_supported, delta, out, amb);
+  QM_HIP(hipGetLastError());
+  std::vector<long long> h_out(shots);
+  std::vector<int> h_amb(shots);
+  QM_HIP(hipMemcpyAsync(h_out.data(), out, sizeof(long long) * shots, hipMemcpyDeviceToHost, st));
+  QM_HIP(hipMemcpyAsync(h_amb.data(), amb, sizeof(int) * shots, hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> flagged;
+  for (int64_t k = 0; k < shots; k++) {
+    outcomes[k] = h_out[k];
+    if (h_amb[k]) flagged.push_back(k);
+  }
+  if (!flagged.empty()) {
+    std::sort(flagged.begin(), flagged.end(),
+              [&](int64_t a, int64_t b) { return uniforms[a] < uniforms[b]; });
+    std::vector<double> fu(flagged.size());
+    for (size_t i = 0; i < flagged.size(); i++) fu[i] = uniforms[flagged[i]];
+    const int64_t nf = (int64_t)flagged.size();
+    QM_HIP(hipMemcpyAsync(draws, fu.data(), sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    if (s->precision == QM_C128)
+      k_serial_resolve<C128><<<1, 64, 0, st>>>((const C128 *)s->data, n, draws, nf, out);
+    else
+      k_serial_resolve<C64><<<1, 64, 0, st>>>((const C64 *)s->data, n, draws, nf, out);
+    QM_HIP(hipGetLastError());
+    QM_HIP(hipMemcpyAsync(h_out.data(), out, sizeof(long long) * nf, hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < nf; i++) outcomes[flagged[i]] = h_out[i];
+  }
+  return (int64_t)flagged.size();
+}
+
+}  // namespace qm

@@ -1,0 +1,519 @@
+// Gate preparation and fused-pass planning (host only).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "gate_math.hpp"
+
+namespace qm {
+
+struct Status;
+[[noreturn]] void fail(int code, const std::string &msg);
+
+static inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+void validate_op(const qm_op &op, int n) {
+  // QuantumCircuit.Add (QuantumCircuit.cs:97-126): range, distinctness, finite angle.
+  if (op.kind < 0 || op.kind > 16) fail(QM_ERR_UNSUPPORTED, "Unsupported gate kind.");
+  if (op.target < 0 || op.target >= n) fail(QM_ERR_RANGE, "qubit: target out of range.");
+  if (op.control != -1) {
+    if (op.control < 0 || op.control >= n) fail(QM_ERR_RANGE, "qubit: control out of range.");
+    if (op.control == op.target) fail(QM_ERR_ARG, "The two qubits must differ.");
+  }
+  if (op.second_control != -1) {
+    if (op.second_control < 0 || op.second_control >= n) fail(QM_ERR_RANGE, "qubit: second control out of range.");
+    if (op.second_control == op.target || op.second_control == op.control)
+      fail(QM_ERR_ARG, "All three qubits must differ.");
+  }
+  if (!std::isfinite(op.angle)) fail(QM_ERR_RANGE, "angle must be finite.");
+  if (op.kind == 10 && op.control < 0) fail(QM_ERR_ARG, "Swap needs two qubits.");
+  if ((op.kind == 9 || op.kind == 16) && op.control < 0) fail(QM_ERR_ARG, "Controlled X needs a control.");
+  if (op.kind == 16 && op.second_control < 0) fail(QM_ERR_ARG, "Toffoli needs two controls.");
+  if (op.kind == 15)
+    for (double v : op.unitary)
+      if (!std::isfinite(v)) fail(QM_ERR_RANGE, "Matrix entries must be finite.");
+}
+
+bool prepare_gate(const qm_op &op, DevGate *g) {
+  std::memset(g, 0, sizeof(*g));
+  uint64_t mask = op.control >= 0 ? (uint64_t)1 << op.control : 0;
+  if (op.second_control >= 0) mask |= (uint64_t)1 << op.second_control;
+  g->target = op.target;
+  g->ctrl_mask = mask;
+  g->swap_other = -1;
+  auto diag1 = [&](double re, double im) {
+    g->kind = G_DIAG1;
+    g->c[0] = 1, g->c[1] = 0, g->c[6] = re, g->c[7] = im;
+  };
+  switch (op.kind) {
+    case 11:  // I
+      return false;
+    case 1:   // X
+    case 9:   // CNot
+    case 16:  // Toffoli
+      g->kind = G_X;
+      return true;
+    case 10:  // Swap(a,b): control = a, target = b; prepared without a control mask
+      g->kind = G_SWAP;
+      g->ctrl_mask = 0;
+      g->swap_other = op.control;
+      return true;
+    case 0:
+      g->kind = G_H;
+      g->c[0] = std::sqrt(0.5);
+      return true;
+    case 2:
+      g->kind = G_Y;
+      return true;
+    case 3:
+      diag1(-1, 0);
+      return true;
+    case 4:
+      diag1(0, 1);
+      return true;
+    case 12:
+      diag1(-0.0, -1);
+      return true;
+    case 5:
+      diag1(std::cos(M_PI / 4), std::sin(M_PI / 4));
+      return true;
+    case 13:
+      diag1(std::cos(-M_PI / 4), std::sin(-M_PI / 4));
+      return true;
+    case 14:
+      diag1(std::cos(op.angle), std::sin(op.angle));
+      return true;
+    case 6:
+      g->kind = G_RX;
+      g->c[0] = std::cos(op.angle / 2);
+      g->c[1] = std::sin(op.angle / 2);
+      return true;
+    case 7:
+      g->kind = G_RY;
+      g->c[0] = std::cos(op.angle / 2);
+      g->c[1] = std::sin(op.angle / 2);
+      return true;
+    case 8:
+      g->kind = G_DIAG;
+      g->c[0] = std::cos(-op.angle / 2), g->c[1] = std::sin(-op.angle / 2);
+      g->c[6] = std::cos(op.angle / 2), g->c[7] = std::sin(op.angle / 2);
+      return true;
+    case 15:
+      g->kind = G_MAT;
+      for (int i = 0; i < 8; i++) g->c[i] = op.unitary[i];
+      return true;
+    default:
+      fail(QM_ERR_UNSUPPORTED, "Unsupported gate kind.");
+  }
+}
+
+double touched_fraction(const DevGate &g) {
+  double f = 1.0 / (double)((uint64_t)1 << popc(g.ctrl_mask));
+  if (g.kind == G_DIAG1 || g.kind == G_SWAP) f *= 0.5;
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// Planner.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PGate {
+  DevGate g;
+  bool pair;        // needs its target as a register bit
+  uint64_t touch;   // target and control qubits
+};
+
+std::vector<PGate> expand(const std::vector<DevGate> &in) {
+  std::vector<PGate> out;
+  out.reserve(in.size() * 2);
+  for (const DevGate &g : in) {
+    if (g.kind == G_SWAP) {
+      // SWAP(a,b) = CX(a->b) CX(b->a) CX(a->b): three exact exchanges.
+      int a = g.swap_other, b = g.target;
+      DevGate x{};
+      x.kind = G_X;
+      x.swap_other = -1;
+      x.target = b, x.ctrl_mask = (uint64_t)1 << a;
+      out.push_back({x, true, ((uint64_t)1 << a) | ((uint64_t)1 << b)});
+      x.target = a, x.ctrl_mask = (uint64_t)1 << b;
+      out.push_back({x, true, ((uint64_t)1 << a) | ((uint64_t)1 << b)});
+      x.target = b, x.ctrl_mask = (uint64_t)1 << a;
+      out.push_back({x, true, ((uint64_t)1 << a) | ((uint64_t)1 << b)});
+      continue;
+    }
+    bool pair = !(g.kind == G_DIAG || g.kind == G_DIAG1);
+    out.push_back({g, pair, ((uint64_t)1 << g.target) | g.ctrl_mask});
+  }
+  return out;
+}
+
+// Rank of bit p among the set bits of m (number of set bits below p).
+inline int rank_in(uint32_t m, int p) { return popc(m & ((1u << p) - 1)); }
+
+void fill_mapping(DevPass &d, int slot, uint32_t rmask, int K, int R) {
+  int r = 0, k = 0;
+  for (int p = 0; p < K; p++) {
+    if (rmask & (1u << p)) {
+      (void)r;
+    } else {
+      d.lanepos[slot][k++] = (uint8_t)p;
+    }
+  }
+  for (int reg = 0; reg < (1 << R); reg++) {
+    uint32_t j = 0;
+    int b = 0;
+    for (int p = 0; p < K; p++)
+      if (rmask & (1u << p)) {
+        if ((reg >> b) & 1) j |= 1u << p;
+        b++;
+      }
+    d.regloc[slot][reg] = (uint16_t)j;
+  }
+}
+
+}  // namespace
+
+std::vector<TileOp> FusedPlan::flat_ops() {
+  std::vector<TileOp> all;
+  for (FusedPass &p : passes) {
+    p.dev.op_offset = (int32_t)all.size();
+    all.insert(all.end(), p.ops.begin(), p.ops.end());
+  }
+  return all;
+}
+
+FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedShape &sh) {
+  FusedPlan plan;
+  plan.shape = sh;
+  plan.n = n;
+  plan.source_gates = (int64_t)gates_in.size();
+  if (n < sh.K) fail(QM_ERR_ARG, "fused passes need at least K qubits");
+  const int K = sh.K, L = sh.L, R = sh.R, c = sh.contiguous;
+  const uint64_t all = n >= 64 ? ~0ull : (((uint64_t)1 << n) - 1);
+  const uint64_t low = ((uint64_t)1 << c) - 1;
+  const uint32_t low_local = (1u << c) - 1;
+
+  std::vector<PGate> G = expand(gates_in);
+  std::vector<int> pending(G.size());
+  for (size_t i = 0; i < G.size(); i++) pending[i] = (int)i;
+
+  while (!pending.empty()) {
+    // 1. Tile qubits: greedy scan with dependency blocking.
+    uint64_t Q = low, blocked = 0;
+    std::vector<int> included, deferred;
+    for (size_t k = 0; k < pending.size(); k++) {
+      int idx = pending[k];
+      const PGate &x = G[idx];
+      if (blocked == all) {
+        deferred.insert(deferred.end(), pending.begin() + k, pending.end());
+        break;
+      }
+      if (x.touch & blocked) {
+        deferred.push_back(idx);
+        blocked |= x.touch;
+        continue;
+      }
+      uint64_t tb = (uint64_t)1 << x.g.target;
+      if (x.pair && !(Q & tb)) {
+        if (popc(Q) < K) {
+          Q |= tb;
+        } else {
+          deferred.push_back(idx);
+          blocked |= x.touch;
+          continue;
+        }
+      }
+      included.push_back(idx);
+    }
+    for (int q = 0; popc(Q) < K && q < n; q++) Q |= (uint64_t)1 << q;
+
+    FusedPass pass;
+    DevPass &d = pass.dev;
+    std::memset(&d, 0, sizeof(d));
+    d.qmask = Q;
+    d.n = n;
+    int qpos[64], local_of[64];
+    for (int q = 0; q < 64; q++) local_of[q] = -1;
+    {
+      int k = 0;
+      for (int q = 0; q < n; q++)
+        if (Q & ((uint64_t)1 << q)) {
+          qpos[k] = q;
+          local_of[q] = k;
+          d.qpos[k] = (uint8_t)q;
+          k++;
+        }
+    }
+    // Runs of non-tile bits, ascending.
+    {
+      int src = 0, nr = 0;
+      for (int q = 0; q < n;) {
+        if (Q & ((uint64_t)1 << q)) {
+          q++;
+          continue;
+        }
+        int start = q;
+        while (q < n && !(Q & ((uint64_t)1 << q))) q++;
+        d.run_src[nr] = (uint8_t)src;
+        d.run_dst[nr] = (uint8_t)start;
+        d.run_len[nr] = (uint8_t)(q - start);
+        src += q - start;
+        nr++;
+      }
+      d.nruns = nr;
+    }
+
+    // 2. Phases: register sets inside the tile, same blocking rule.
+    std::vector<int> remaining = included, overflow;
+    std::vector<uint32_t> rsets;
+    std::vector<std::vector<int>> phase_gates;
+    while (!remaining.empty()) {
+      if ((int)rsets.size() == kMaxPhases) {
+        overflow = remaining;
+        break;
+      }
+      uint32_t rset = 0;
+      uint64_t pblocked = 0;
+      std::vector<int> here, next;
+      for (int idx : remaining) {
+        const PGate &x = G[idx];
+        if (x.touch & pblocked) {
+          next.push_back(idx);
+          pblocked |= x.touch;
+          continue;
+        }
+        if (x.pair) {
+          uint32_t lb = 1u << local_of[x.g.target];
+          if (!(rset & lb)) {
+            if (popc(rset) < R) {
+              rset |= lb;
+            } else {
+              next.push_back(idx);
+              pblocked |= x.touch;
+              continue;
+            }
+          }
+        }
+        here.push_back(idx);
+      }
+      // Fill the register set from the top local positions (keeps the
+      // contiguous low bits on lanes, so loads and stores stay coalesced).
+      for (int p = K - 1; popc(rset) < R && p >= 0; p--)
+        if (!(rset & (1u << p)) && !(low_local & (1u << p))) rset |= 1u << p;
+      for (int p = K - 1; popc(rset) < R && p >= 0; p--) rset |= 1u << p;
+      rsets.push_back(rset);
+      phase_gates.push_back(here);
+      remaining.swap(next);
+    }
+    // Gates not placed go back to pending, in original order.
+    if (!overflow.empty()) {
+      deferred.insert(deferred.end(), overflow.begin(), overflow.end());
+      std::sort(deferred.begin(), deferred.end());
+    }
+
+    // 3. Mappings and ops.
+    uint32_t io_rmask = 0;
+    for (int p = K - 1; popc(io_rmask) < R; p--) io_rmask |= 1u << p;
+    d.nphases = (int32_t)rsets.size();
+    for (int ph = 0; ph < d.nphases; ph++) fill_mapping(d, ph, rsets[ph], K, R);
+    fill_mapping(d, kIo, io_rmask, K, R);
+    d.load_direct = (rsets.front() & low_local) == 0;
+    d.store_direct = (rsets.back() & low_local) == 0;
+    pass.transposes = (d.nphases - 1) + (d.load_direct ? 0 : 1) + (d.store_direct ? 0 : 1);
+    auto set_io = [&](int slot, uint64_t *reg_off, uint8_t *lane_gbit) {
+      for (int reg = 0; reg < (1 << R); reg++) {
+        uint32_t j = d.regloc[slot][reg];
+        uint64_t g = 0;
+        for (int p = 0; p < K; p++)
+          if (j & (1u << p)) g |= (uint64_t)1 << qpos[p];
+        reg_off[reg] = g;
+      }
+      for (int k = 0; k < L; k++) lane_gbit[k] = (uint8_t)qpos[d.lanepos[slot][k]];
+    };
+    set_io(d.load_direct ? 0 : kIo, d.load_reg_off, d.load_lane_gbit);
+    set_io(d.store_direct ? d.nphases - 1 : kIo, d.store_reg_off, d.store_lane_gbit);
+
+    for (int ph = 0; ph < d.nphases; ph++) {
+      uint32_t rset = rsets[ph];
+      d.phase_op_begin[ph] = (int32_t)pass.ops.size();
+      for (int idx : phase_gates[ph]) {
+        const PGate &x = G[idx];
+        TileOp t;
+        std::memset(&t, 0, sizeof(t));
+        t.kind = x.g.kind;
+        t.rbit = -1;
+        for (int i = 0; i < 8; i++) t.c[i] = x.g.c[i];
+        for (int q = 0; q < n; q++) {
+          if (!(x.g.ctrl_mask & ((uint64_t)1 << q))) continue;
+          int lp = local_of[q];
+          if (lp < 0)
+            t.out_cmask |= (uint64_t)1 << q;
+          else if (rset & (1u << lp))
+            t.reg_cmask |= 1u << rank_in(rset, lp);
+          else
+            t.lane_cmask |= 1u << rank_in(~rset & ((1u << K) - 1), lp);
+        }
+        int lp = local_of[x.g.target];
+        if (x.pair) {
+          t.rbit = rank_in(rset, lp);
+        } else if (lp < 0) {
+          t.dout = (uint64_t)1 << x.g.target;
+        } else if (rset & (1u << lp)) {
+          t.dreg = 1u << rank_in(rset, lp);
+        } else {
+          t.dlane = 1u << rank_in(~rset & ((1u << K) - 1), lp);
+        }
+        pass.ops.push_back(t);
+        pass.gates++;
+      }
+      d.phase_op_count[ph] = (int32_t)pass.ops.size() - d.phase_op_begin[ph];
+    }
+    plan.passes.push_back(std::move(pass));
+    pending.swap(deferred);
+  }
+  return plan;
+}
+
+// ---------------------------------------------------------------------------
+// Host references.
+// ---------------------------------------------------------------------------
+namespace {
+
+inline void apply_pair_host(int kind, const double *c, double &xr, double &xi, double &yr,
+                            double &yi) {
+  switch (kind) {
+    case G_MAT: pair_mat(c, xr, xi, yr, yi); break;
+    case G_RX: pair_rx(c[0], c[1], xr, xi, yr, yi); break;
+    case G_RY: pair_ry(c[0], c[1], xr, xi, yr, yi); break;
+    case G_H: pair_h(c[0], xr, xi, yr, yi); break;
+    case G_Y: pair_y(xr, xi, yr, yi); break;
+    case G_X: std::swap(xr, yr), std::swap(xi, yi); break;
+    default: break;
+  }
+}
+
+}  // namespace
+
+void apply_gate_host(const DevGate &g, double *s, int n) {
+  const uint64_t len = (uint64_t)1 << n;
+  const uint64_t bit = (uint64_t)1 << g.target;
+  if (g.kind == G_DIAG || g.kind == G_DIAG1) {
+    for (uint64_t i = 0; i < len; i++) {
+      if ((i & g.ctrl_mask) != g.ctrl_mask) continue;
+      if (i & bit)
+        mul_phase(g.c[6], g.c[7], s[2 * i], s[2 * i + 1]);
+      else if (g.kind == G_DIAG)
+        mul_phase(g.c[0], g.c[1], s[2 * i], s[2 * i + 1]);
+    }
+    return;
+  }
+  for (uint64_t i = 0; i < len; i++) {
+    if (i & bit) continue;
+    if ((i & g.ctrl_mask) != g.ctrl_mask) continue;
+    uint64_t j = i | bit;
+    if (g.kind == G_SWAP) {
+      uint64_t ob = (uint64_t)1 << g.swap_other;
+      if (!(i & ob)) continue;
+      j ^= ob;
+      std::swap(s[2 * i], s[2 * j]);
+      std::swap(s[2 * i + 1], s[2 * j + 1]);
+      continue;
+    }
+    apply_pair_host(g.kind, g.c, s[2 * i], s[2 * i + 1], s[2 * j], s[2 * j + 1]);
+  }
+}
+
+void apply_plan_host(const FusedPlan &plan, double *s, int n) {
+  const int K = plan.shape.K, L = plan.shape.L, R = plan.shape.R;
+  const int regs = 1 << R, threads = 1 << L;
+  std::vector<double> lds(2 * ((size_t)1 << K));
+  std::vector<double> v(2 * (size_t)regs * threads);
+  for (const FusedPass &p : plan.passes) {
+    const DevPass &d = p.dev;
+    const uint64_t tiles = (uint64_t)1 << (n - K);
+    for (uint64_t tau = 0; tau < tiles; tau++) {
+      uint64_t base = 0;
+      for (int r = 0; r < d.nruns; r++)
+        base |= ((tau >> d.run_src[r]) & (((uint64_t)1 << d.run_len[r]) - 1)) << d.run_dst[r];
+      auto lane_local = [&](int slot, int t) {
+        uint32_t j = 0;
+        for (int k = 0; k < L; k++)
+          if ((t >> k) & 1) j |= 1u << d.lanepos[slot][k];
+        return j;
+      };
+      auto transpose = [&](int from, int to) {
+        for (int t = 0; t < threads; t++)
+          for (int r = 0; r < regs; r++) {
+            uint32_t j = lane_local(from, t) | d.regloc[from][r];
+            lds[2 * j] = v[2 * (t * regs + r)];
+            lds[2 * j + 1] = v[2 * (t * regs + r) + 1];
+          }
+        for (int t = 0; t < threads; t++)
+          for (int r = 0; r < regs; r++) {
+            uint32_t j = lane_local(to, t) | d.regloc[to][r];
+            v[2 * (t * regs + r)] = lds[2 * j];
+            v[2 * (t * regs + r) + 1] = lds[2 * j + 1];
+          }
+      };
+      for (int t = 0; t < threads; t++) {
+        uint64_t lo = 0;
+        for (int k = 0; k < L; k++)
+          if ((t >> k) & 1) lo |= (uint64_t)1 << d.load_lane_gbit[k];
+        for (int r = 0; r < regs; r++) {
+          uint64_t g = base | lo | d.load_reg_off[r];
+          v[2 * (t * regs + r)] = s[2 * g];
+          v[2 * (t * regs + r) + 1] = s[2 * g + 1];
+        }
+      }
+      int cur = d.load_direct ? 0 : kIo;
+      for (int ph = 0; ph < d.nphases; ph++) {
+        if (cur != ph) transpose(cur, ph);
+        cur = ph;
+        for (int o = 0; o < d.phase_op_count[ph]; o++) {
+          const TileOp &op = p.ops[d.phase_op_begin[ph] + o];
+          bool out_ok = (base & op.out_cmask) == op.out_cmask;
+          if (!out_ok) continue;
+          for (int t = 0; t < threads; t++) {
+            if ((t & op.lane_cmask) != op.lane_cmask) continue;
+            double *vt = &v[2 * (size_t)t * regs];
+            if (op.rbit >= 0) {
+              for (int r = 0; r < regs; r++) {
+                if (r & (1 << op.rbit)) continue;
+                if ((r & op.reg_cmask) != op.reg_cmask) continue;
+                int r1 = r | (1 << op.rbit);
+                apply_pair_host(op.kind, op.c, vt[2 * r], vt[2 * r + 1], vt[2 * r1], vt[2 * r1 + 1]);
+              }
+            } else {
+              bool hi_fixed = ((t & op.dlane) != 0) || ((base & op.dout) != 0);
+              for (int r = 0; r < regs; r++) {
+                if ((r & op.reg_cmask) != op.reg_cmask) continue;
+                bool hi = hi_fixed || ((r & op.dreg) != 0);
+                if (hi)
+                  mul_phase(op.c[6], op.c[7], vt[2 * r], vt[2 * r + 1]);
+                else if (op.kind == G_DIAG)
+                  mul_phase(op.c[0], op.c[1], vt[2 * r], vt[2 * r + 1]);
+              }
+            }
+          }
+        }
+      }
+      if (!d.store_direct) transpose(cur, kIo);
+      for (int t = 0; t < threads; t++) {
+        uint64_t lo = 0;
+        for (int k = 0; k < L; k++)
+          if ((t >> k) & 1) lo |= (uint64_t)1 << d.store_lane_gbit[k];
+        for (int r = 0; r < regs; r++) {
+          uint64_t g = base | lo | d.store_reg_off[r];
+          s[2 * g] = v[2 * (t * regs + r)];
+          s[2 * g + 1] = v[2 * (t * regs + r) + 1];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace qm

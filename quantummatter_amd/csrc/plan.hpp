@@ -1,0 +1,134 @@
+// Host-side gate preparation and fused-pass planning for libqmhip.so.
+//
+// Pure C++ (no HIP types) so the planner is unit-tested on the CPU.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "qm_hip.h"
+
+namespace qm {
+
+// Device gate kinds.  Each is an arithmetic specialisation of the reference's
+// 2x2 update (CpuStateVectorBackend.cs:93-177) that rounds exactly as the
+// general complex form does for that coefficient pattern (no FMA anywhere).
+enum GKind : int32_t {
+  G_MAT = 0,    // general 2x2 complex matrix (A,B,C,D)
+  G_RX = 1,     // (c, -i s; -i s, c): c[0]=c, c[1]=s
+  G_RY = 2,     // (c, -s; s, c):      c[0]=c, c[1]=s
+  G_H = 3,      // (h, h; h, -h):      c[0]=h
+  G_Y = 4,      // (0, -i; i, 0): exact permutation with sign
+  G_X = 5,      // exchange (X, CNOT, Toffoli)
+  G_DIAG = 6,   // diag(A, D) on both halves: A = c[0..1], D = c[6..7]
+  G_DIAG1 = 7,  // diag(1, D): A is exactly 1, only the one-half changes
+  G_SWAP = 8,   // exchange of (a=1,t=0) with (a=0,t=1); swap_other = a
+};
+
+struct DevGate {
+  int32_t kind;
+  int32_t target;
+  int32_t swap_other;  // G_SWAP: the other qubit; else -1
+  int32_t pad;
+  uint64_t ctrl_mask;
+  double c[8];  // coefficients, layout per kind
+};
+
+// Validates one GateOperation as QuantumCircuit.Add does (QuantumCircuit.cs:97-126);
+// throws qm::Status on failure.
+void validate_op(const qm_op &op, int qubits);
+
+// PreparedStateVectorGate.Prepare (PreparedStateVectorGate.cs:13-70), refined
+// into a device gate kind.  Returns false for identity (skipped, as every
+// reference backend skips StateVectorKernelKind.Identity).
+bool prepare_gate(const qm_op &op, DevGate *out);
+
+// Fraction of the state a gate reads and writes.
+double touched_fraction(const DevGate &g);
+
+// ---------------------------------------------------------------------------
+// Fused tile passes.
+//
+// A pass reads every amplitude once, applies a run of gates in registers, and
+// writes it back.  The state is cut into tiles of 2^K amplitudes spanned by K
+// "tile qubits" Q, which always include the lowest `contiguous` qubits so
+// every global access is a >=128-byte contiguous run.  Each of the 2^L threads
+// of a tile holds 2^R amplitudes (K = L + R); which R tile bits are register
+// bits changes between phases by an LDS transpose.  A pair gate's target must
+// be a register bit of its phase; a diagonal gate's target and every control
+// may be a register bit, a lane bit or a qubit outside the tile.
+// ---------------------------------------------------------------------------
+constexpr int kMaxK = 14;
+constexpr int kMaxPhases = 12;
+constexpr int kMaxRegs = 32;
+constexpr int kIo = kMaxPhases;  // mapping slot of the I/O layout
+
+struct TileOp {
+  int32_t kind;         // GKind (swaps are planned as three exchanges)
+  int32_t rbit;         // register bit of the target for pair kinds, else -1
+  uint32_t reg_cmask;   // control bits that are register bits (mask on register index)
+  uint32_t lane_cmask;  // control bits that are lane bits (mask on thread index)
+  uint64_t out_cmask;   // control bits outside the tile (global mask)
+  uint32_t dreg;        // diagonal target as a register-index mask (0 if not a register bit)
+  uint32_t dlane;       // diagonal target as a thread-index mask (0 if not a lane bit)
+  uint64_t dout;        // diagonal target as a global mask outside the tile (0 if inside)
+  double c[8];
+};
+static_assert(sizeof(TileOp) == 104, "TileOp layout");
+
+struct DevPass {
+  uint64_t qmask;                     // global mask of the tile qubits
+  uint64_t load_reg_off[kMaxRegs];    // global offset of register r under the load mapping
+  uint64_t store_reg_off[kMaxRegs];   // ... under the store mapping
+  int32_t n;
+  int32_t nphases;
+  int32_t load_direct;   // 1: load straight into phase 0's mapping
+  int32_t store_direct;  // 1: store straight from the last phase's mapping
+  int32_t nruns;         // runs of non-tile bits (tile index -> global base)
+  int32_t op_offset;     // first TileOp of this pass in the program
+  int32_t phase_op_begin[kMaxPhases];  // relative to op_offset
+  int32_t phase_op_count[kMaxPhases];
+  uint16_t regloc[kMaxPhases + 1][kMaxRegs];  // local index of register r per mapping
+  uint8_t lanepos[kMaxPhases + 1][16];        // local position of thread bit k per mapping
+  uint8_t load_lane_gbit[16];                 // global bit of thread bit k under the load mapping
+  uint8_t store_lane_gbit[16];
+  uint8_t run_src[16];
+  uint8_t run_dst[16];
+  uint8_t run_len[16];
+  uint8_t qpos[16];  // tile qubits ascending
+};
+
+struct FusedShape {
+  int K;           // tile bits
+  int L;           // lane bits (threads = 2^L)
+  int R;           // register bits
+  int contiguous;  // lowest qubits always in the tile
+};
+
+struct FusedPass {
+  DevPass dev;
+  std::vector<TileOp> ops;  // phases contiguous
+  int gates = 0;            // source gates applied (after swap expansion)
+  int transposes = 0;       // LDS round trips
+};
+
+struct FusedPlan {
+  FusedShape shape{};
+  int n = 0;
+  std::vector<FusedPass> passes;
+  int64_t source_gates = 0;
+  // Concatenated ops of all passes; fills DevPass::op_offset.
+  std::vector<TileOp> flat_ops();
+};
+
+// Plans prepared gates (identity removed) for an n-qubit state, n >= shape.K.
+FusedPlan plan_fused(const std::vector<DevGate> &gates, int n, const FusedShape &shape);
+
+// Host reference of the kernels, used by the planner's CPU tests: applies a
+// plan / one gate to an interleaved complex128 state with the kernels'
+// arithmetic and indexing.
+void apply_plan_host(const FusedPlan &plan, double *state, int n);
+void apply_gate_host(const DevGate &g, double *state, int n);
+
+}  // namespace qm

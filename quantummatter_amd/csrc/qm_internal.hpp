@@ -1,0 +1,108 @@
+// Internal types shared by the host runtime and the kernels of libqmhip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "qm_hip.h"
+#include "plan.hpp"
+
+namespace qm {
+
+// Thread-local error text behind qm_last_error().
+void set_error(const std::string &msg);
+void clear_error();
+
+// A status with its message, carried through the runtime by exception and
+// turned into a return code at the C boundary.
+struct Status {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg);
+void hip_check(hipError_t e, const char *what);
+
+#define QM_HIP(call) ::qm::hip_check((call), #call)
+
+struct Ctx {
+  int device = 0;
+  bool profiling = false;
+  int64_t device_budget = 0;   // 0: device capacity
+  int64_t staging_budget = 0;  // 0: unlimited
+  int64_t device_memory = 0;
+  int compute_units = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  std::recursive_mutex lock;  // serialises calls on one context
+  int64_t resident_bytes = 0;
+  // Reduction scratch (grown on demand, never shrunk while the context lives).
+  double *d_partial = nullptr;
+  int64_t partial_cap = 0;  // doubles
+  int64_t *d_ipartial = nullptr;
+  int64_t ipartial_cap = 0;
+  double *h_pinned = nullptr;  // small pinned readback area
+  int64_t pinned_cap = 0;
+  // Device copy of fused-pass programs.
+  void *d_prog = nullptr;
+  int64_t prog_cap = 0;
+  bool disposed = false;
+};
+
+struct State {
+  Ctx *ctx = nullptr;
+  int qubits = 0;
+  int precision = QM_C128;
+  int64_t length = 0;  // amplitudes
+  void *data = nullptr;
+  int64_t bytes = 0;
+  int64_t boundary_fallbacks = 0;
+};
+
+inline int64_t amp_bytes(int precision) { return precision == QM_C128 ? 16 : 8; }
+
+// Scratch management on the context's stream.
+double *ctx_partials(Ctx *c, int64_t n);
+int64_t *ctx_ipartials(Ctx *c, int64_t n);
+double *ctx_pinned(Ctx *c, int64_t n);
+void *ctx_program(Ctx *c, int64_t bytes);
+
+// ---- kernel launchers (kernels.hip) ---------------------------------------
+void launch_gate(State *s, const DevGate &g, hipStream_t st);
+void launch_fused_pass(State *s, const FusedPass &p, const void *d_pass, const void *d_ops,
+                       hipStream_t st);
+FusedShape fused_shape(int precision);
+void launch_set_basis(State *s, int64_t index, hipStream_t st);  // |index>
+void launch_scale_pairs(State *s, int target, bool jump, double scale, double jump_scale,
+                        double stay_scale, hipStream_t st);
+void launch_collapse_subset(State *s, uint64_t mask, uint64_t pattern, double norm, hipStream_t st);
+void launch_convert(const State *src, State *dst, hipStream_t st);
+
+// Reductions.  Results land in host memory (synchronous on the stream).
+struct MassResult {
+  double total;
+  int64_t last_supported;
+};
+MassResult reduce_mass(State *s, hipStream_t st);
+// Masked sum of probabilities: sum over i with (i & mask) == pattern.
+double reduce_masked_mass(State *s, uint64_t mask, uint64_t pattern, hipStream_t st);
+// Probability masses of target bit 0 and 1.
+void reduce_bit_masses(State *s, int target, double *m0, double *m1, hipStream_t st);
+// Kahan-compensated sequential masses in the reference's order (one lane).
+void serial_bit_masses(State *s, int target, double *m0, double *m1, hipStream_t st);
+// Un-normalised expectation numerators for a set of Pauli terms.
+void reduce_pauli(State *s, const qm_pauli *terms, int64_t n_terms, double *numerators,
+                  hipStream_t st);
+void reduce_inner(State *a, State *b, double *re, double *im, hipStream_t st);
+
+// CDF search for a batch of draws with the strict-upper-bound rule; resolves
+// draws within the rounding margin of a boundary with the sequential order.
+// Returns the number of margin fallbacks.
+int64_t cdf_search(State *s, const double *uniforms, int64_t n, int64_t *outcomes, hipStream_t st,
+                   double *total_out = nullptr);
+
+}  // namespace qm
