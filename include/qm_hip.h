@@ -128,6 +128,12 @@ int qm_ctx_create(const qm_ctx_options *options, qm_ctx **out);
 int qm_ctx_destroy(qm_ctx *ctx);
 /* Device memory currently held by this context's states (bytes). */
 int qm_ctx_resident_bytes(qm_ctx *ctx, int64_t *bytes);
+/* No reference counterpart (the reference has no fusion): the fused schedule's
+ * tile shape per precision -- 2^tile_bits amplitudes per tile, 2^lane_bits
+ * threads per workgroup.  Only compiled shapes are accepted (QM_ERR_UNSUPPORTED
+ * otherwise).  Set by the caller's tuning or by qm_autotune. */
+int qm_ctx_set_fused_shape(qm_ctx *ctx, int32_t precision, int32_t tile_bits, int32_t lane_bits);
+int qm_ctx_get_fused_shape(qm_ctx *ctx, int32_t precision, int32_t *tile_bits, int32_t *lane_bits);
 
 /* ---- drop-in whole-circuit execution ------------------------------------ */
 /* IStateVectorBackend.Execute (src/QuantumMatter.Core/IStateVectorBackend.cs:13)

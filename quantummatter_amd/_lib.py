@@ -42,6 +42,8 @@ SIGNATURES = {
     "qm_ctx_create": [P, P],
     "qm_ctx_destroy": [P],
     "qm_ctx_resident_bytes": [P, P],
+    "qm_ctx_set_fused_shape": [P, I32, I32, I32],
+    "qm_ctx_get_fused_shape": [P, I32, P, P],
     "qm_execute": [P, I32, P, I64, P, I64, P, P],
     "qm_state_create": [P, I32, I32, P],
     "qm_state_destroy": [P],

@@ -127,6 +127,15 @@ class HipStateVectorBackend:
             call("qm_execute", self._handle(), circuit.QubitCount, ptr(ops), len(ops), ptr(amplitudes),
                  amplitudes.size, _cancel_ptr(cancellation), None if prof is None else C.byref(prof))
 
+    def SetFusedShape(self, precision: int, tile_bits: int, lane_bits: int) -> None:
+        """Selects a compiled tile shape for the fused schedule (a tuning knob)."""
+        call("qm_ctx_set_fused_shape", self._handle(), precision, tile_bits, lane_bits)
+
+    def GetFusedShape(self, precision: int):
+        k, l = C.c_int32(), C.c_int32()
+        call("qm_ctx_get_fused_shape", self._handle(), precision, C.byref(k), C.byref(l))
+        return k.value, l.value
+
     def ResidentBytes(self) -> int:
         b = C.c_int64()
         call("qm_ctx_resident_bytes", self._handle(), C.byref(b))

@@ -141,7 +141,7 @@ static ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int sche
   Ctx *c = s->ctx;
   ExecStats st;
   const double state_bytes = (double)s->bytes;
-  const FusedShape shape = fused_shape(s->precision);
+  const FusedShape shape = c->shape[s->precision];
   if (schedule == QM_SCHED_FUSED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     std::vector<TileOp> flat = plan.flat_ops();
@@ -155,7 +155,7 @@ static ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int sche
     QM_HIP(hipMemcpyAsync(prog, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
     for (size_t i = 0; i < plan.passes.size(); i++) {
       check_cancel(cancel);
-      launch_fused_pass(s, plan.passes[i], prog + i * sizeof(DevPass), prog + pass_bytes, c->stream);
+      launch_fused_pass(s, shape, prog + i * sizeof(DevPass), prog + pass_bytes, c->stream);
       st.launches++;
       st.passes++;
       st.bytes += 2 * state_bytes;
@@ -678,13 +678,36 @@ int qm_state_boundary_fallbacks(qm_state *h, int64_t *count) {
   });
 }
 
+int qm_ctx_set_fused_shape(qm_ctx *h, int32_t precision, int32_t tile_bits, int32_t lane_bits) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    use_device(c);
+    if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+    const int R = tile_bits - lane_bits;
+    if (!fused_shape_supported(precision, lane_bits, R))
+      fail(QM_ERR_UNSUPPORTED, "fused tile shape is not compiled");
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    c->shape[precision] = FusedShape{tile_bits, lane_bits, R, precision == QM_C128 ? 3 : 4};
+  });
+}
+
+int qm_ctx_get_fused_shape(qm_ctx *h, int32_t precision, int32_t *tile_bits, int32_t *lane_bits) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    use_device(c);
+    if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+    if (tile_bits) *tile_bits = c->shape[precision].K;
+    if (lane_bits) *lane_bits = c->shape[precision].L;
+  });
+}
+
 int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n_ops,
                   int32_t *passes, int32_t *transposes, int64_t *gates) {
   return guarded([&] {
     if (qubits < 1 || qubits > 34) fail(QM_ERR_RANGE, "qubits must be between 1 and 34.");
     if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
     std::vector<DevGate> g = prepare_all(ops, n_ops, qubits);
-    const FusedShape shape = fused_shape(precision);
+    const FusedShape shape = default_fused_shape(precision);
     int32_t np = 0, nt = 0;
     if (qubits >= shape.K && g.size() > 1) {
       FusedPlan plan = plan_fused(g, qubits, shape);

@@ -1,0 +1,254 @@
+// Fused tile-pass kernel of libqmhip.so (gfx950).
+//
+// One workgroup owns one tile of 2^K amplitudes at a time (grid-stride over
+// tiles).  Each of the 2^L threads keeps 2^R amplitudes in registers; a gate
+// whose target is a register bit runs entirely in registers, a diagonal gate
+// runs wherever its target lives, and the set of register bits changes
+// between phases by an LDS transpose.  The pass reads and writes every
+// amplitude of the state exactly once.  The plan (plan.cpp) supplies, per pass,
+// the tile qubits, the register set of each phase and the ops of each phase.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "dev_types.hpp"
+#include "gate_math.hpp"
+#include "qm_internal.hpp"
+
+namespace qm {
+
+template <typename F, int REGS, int B, int KIND>
+__device__ __forceinline__ void reg_pair(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, F c0, F c1, F c2,
+                                         F c3, F c4, F c5, F c6, F c7) {
+#pragma unroll
+  for (int r = 0; r < REGS; r++) {
+    if (r & (1 << B)) continue;
+    if (((uint32_t)r & m) != m) continue;
+    const int r1 = r | (1 << B);
+    if constexpr (KIND == G_MAT) {
+      const F c[8] = {c0, c1, c2, c3, c4, c5, c6, c7};
+      pair_mat<F>(c, vr[r], vi[r], vr[r1], vi[r1]);
+    } else if constexpr (KIND == G_RX) {
+      pair_rx<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
+    } else if constexpr (KIND == G_RY) {
+      pair_ry<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
+    } else if constexpr (KIND == G_H) {
+      pair_h<F>(c0, vr[r], vi[r], vr[r1], vi[r1]);
+    } else if constexpr (KIND == G_Y) {
+      pair_y<F>(vr[r], vi[r], vr[r1], vi[r1]);
+    } else {  // G_X
+      const F tr = vr[r], ti = vi[r];
+      vr[r] = vr[r1];
+      vi[r] = vi[r1];
+      vr[r1] = tr;
+      vi[r1] = ti;
+    }
+  }
+}
+
+template <typename F, int REGS, int KIND>
+__device__ __forceinline__ void reg_pair_bit(int b, F (&vr)[REGS], F (&vi)[REGS], uint32_t m, F c0,
+                                             F c1, F c2, F c3, F c4, F c5, F c6, F c7) {
+  switch (b) {
+    case 0: reg_pair<F, REGS, 0, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
+    case 1: reg_pair<F, REGS, 1, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
+    case 2: reg_pair<F, REGS, 2, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
+    case 3: reg_pair<F, REGS, 3, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
+    default:
+      if constexpr (REGS > 16) reg_pair<F, REGS, 4, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7);
+      break;
+  }
+}
+
+template <typename F, int REGS>
+__device__ __forceinline__ void apply_tile_op(const TileOp *__restrict__ op, F (&vr)[REGS],
+                                              F (&vi)[REGS], uint32_t t, uint64_t base) {
+  const int kind = op->kind;
+  const int rbit = op->rbit;
+  const uint32_t reg_cmask = op->reg_cmask;
+  const uint32_t lane_cmask = op->lane_cmask;
+  const uint64_t out_cmask = op->out_cmask;
+  if ((base & out_cmask) != out_cmask) return;
+  if ((t & lane_cmask) != lane_cmask) return;
+  const F c0 = (F)op->c[0], c1 = (F)op->c[1], c6 = (F)op->c[6], c7 = (F)op->c[7];
+  if (rbit >= 0) {
+    switch (kind) {
+      case G_MAT: {
+        const F c2 = (F)op->c[2], c3 = (F)op->c[3], c4 = (F)op->c[4], c5 = (F)op->c[5];
+        reg_pair_bit<F, REGS, G_MAT>(rbit, vr, vi, reg_cmask, c0, c1, c2, c3, c4, c5, c6, c7);
+        break;
+      }
+      case G_RX: reg_pair_bit<F, REGS, G_RX>(rbit, vr, vi, reg_cmask, c0, c1, 0, 0, 0, 0, 0, 0); break;
+      case G_RY: reg_pair_bit<F, REGS, G_RY>(rbit, vr, vi, reg_cmask, c0, c1, 0, 0, 0, 0, 0, 0); break;
+      case G_H: reg_pair_bit<F, REGS, G_H>(rbit, vr, vi, reg_cmask, c0, 0, 0, 0, 0, 0, 0, 0); break;
+      case G_Y: reg_pair_bit<F, REGS, G_Y>(rbit, vr, vi, reg_cmask, 0, 0, 0, 0, 0, 0, 0, 0); break;
+      default: reg_pair_bit<F, REGS, G_X>(rbit, vr, vi, reg_cmask, 0, 0, 0, 0, 0, 0, 0, 0); break;
+    }
+    return;
+  }
+  // Diagonal: D where the target bit is one; A (DIAG) or nothing (DIAG1) where zero.
+  const uint32_t dreg = op->dreg;
+  const bool hi_fixed = ((base & op->dout) != 0) || ((t & op->dlane) != 0);
+  if (kind == G_DIAG) {
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
+      const bool hi = hi_fixed || (((uint32_t)r & dreg) != 0);
+      mul_phase<F>(hi ? c6 : c0, hi ? c7 : c1, vr[r], vi[r]);
+    }
+  } else if (op->dlane == 0) {
+    // The target is a register bit or outside the tile: a uniform choice per register.
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
+      if (hi_fixed || (((uint32_t)r & dreg) != 0)) mul_phase<F>(c6, c7, vr[r], vi[r]);
+    }
+  } else if (hi_fixed) {
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
+      mul_phase<F>(c6, c7, vr[r], vi[r]);
+    }
+  }
+}
+
+template <int L>
+__device__ __forceinline__ uint32_t lane_swizzled(const DevPass &d, int slot, uint32_t t) {
+  uint32_t j = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++)
+    if ((t >> k) & 1) j |= 1u << d.lanepos[slot][k];
+  return lds_swizzle(j);
+}
+
+template <typename T, typename F, int L, int REGS>
+__device__ __forceinline__ void lds_transpose(T *__restrict__ lds, const DevPass &d, int from, int to,
+                                              uint32_t t, F (&vr)[REGS], F (&vi)[REGS]) {
+  __syncthreads();
+  const uint32_t jf = lane_swizzled<L>(d, from, t);
+#pragma unroll
+  for (int r = 0; r < REGS; r++) {
+    T a;
+    a.re = vr[r];
+    a.im = vi[r];
+    lds[jf ^ d.regswz[from][r]] = a;
+  }
+  __syncthreads();
+  const uint32_t jt = lane_swizzled<L>(d, to, t);
+#pragma unroll
+  for (int r = 0; r < REGS; r++) {
+    const T a = lds[jt ^ d.regswz[to][r]];
+    vr[r] = a.re;
+    vi[r] = a.im;
+  }
+}
+
+template <typename T, int L, int R>
+__global__ __launch_bounds__(1 << L, 2) void k_fused(T *__restrict__ s, const DevPass *__restrict__ pass,
+                                                     const TileOp *__restrict__ prog, uint64_t tiles) {
+  using F = typename Real<T>::type;
+  constexpr int REGS = 1 << R;
+  extern __shared__ char lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
+  const DevPass &d = *pass;
+  const uint32_t t = threadIdx.x;
+  uint64_t load_lane = 0, store_lane = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    if ((t >> k) & 1) {
+      load_lane |= 1ull << d.load_lane_gbit[k];
+      store_lane |= 1ull << d.store_lane_gbit[k];
+    }
+  }
+  const int nph = d.nphases;
+  const bool load_direct = d.load_direct != 0, store_direct = d.store_direct != 0;
+  const TileOp *__restrict__ ops = prog + d.op_offset;
+  F vr[REGS], vi[REGS];
+
+  for (uint64_t tau = blockIdx.x; tau < tiles; tau += gridDim.x) {
+    uint64_t base = 0;
+    for (int k = 0; k < d.nruns; k++)
+      base |= ((tau >> d.run_src[k]) & ((1ull << d.run_len[k]) - 1)) << d.run_dst[k];
+    const T *src = s + (base | load_lane);
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      const T a = src[d.load_reg_off[r]];
+      vr[r] = a.re;
+      vi[r] = a.im;
+    }
+    int cur = load_direct ? 0 : kIo;
+    for (int ph = 0; ph < nph; ph++) {
+      if (cur != ph) lds_transpose<T, F, L, REGS>(lds, d, cur, ph, t, vr, vi);
+      cur = ph;
+      const TileOp *__restrict__ po = ops + d.phase_op_begin[ph];
+      const int cnt = d.phase_op_count[ph];
+      for (int o = 0; o < cnt; o++) apply_tile_op<F, REGS>(po + o, vr, vi, t, base);
+    }
+    if (!store_direct) lds_transpose<T, F, L, REGS>(lds, d, cur, kIo, t, vr, vi);
+    T *dst = s + (base | store_lane);
+#pragma unroll
+    for (int r = 0; r < REGS; r++) {
+      T a;
+      a.re = vr[r];
+      a.im = vi[r];
+      dst[d.store_reg_off[r]] = a;
+    }
+  }
+}
+
+template <typename T, int L, int R>
+static void launch_fused_t(State *st, const void *d_pass, const void *d_ops, hipStream_t stream) {
+  constexpr int K = L + R;
+  const uint64_t tiles = 1ull << (st->qubits - K);
+  const size_t lds = sizeof(T) << K;
+  static int resident = 0;  // workgroups per CU for this instance (one device per process)
+  if (resident == 0) {
+    QM_HIP(hipFuncSetAttribute((const void *)k_fused<T, L, R>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    QM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused<T, L, R>, 1 << L, lds));
+    resident = std::max(per_cu, 1);
+  }
+  // Persistent grid: every workgroup resident, each walking its share of tiles.
+  const uint64_t want = (uint64_t)st->ctx->compute_units * resident;
+  const int blocks = (int)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
+  k_fused<T, L, R><<<blocks, 1 << L, lds, stream>>>((T *)st->data, (const DevPass *)d_pass,
+                                                     (const TileOp *)d_ops, tiles);
+  QM_HIP(hipGetLastError());
+}
+
+// Compiled tile shapes (lane bits L, register bits R): K = L + R tile qubits.
+#define QM_FUSED_SHAPES(X) \
+  X(C128, 8, 4) X(C128, 7, 4) X(C128, 9, 3) X(C128, 9, 4) X(C128, 8, 3) \
+  X(C64, 8, 5) X(C64, 8, 4) X(C64, 9, 4) X(C64, 7, 5) X(C64, 9, 3)
+
+bool fused_shape_supported(int precision, int L, int R) {
+#define QM_SUPPORTED(T, l, r) \
+  if ((precision == QM_C128) == std::is_same<T, C128>::value && L == l && R == r) return true;
+  QM_FUSED_SHAPES(QM_SUPPORTED)
+#undef QM_SUPPORTED
+  return false;
+}
+
+FusedShape default_fused_shape(int precision) {
+  // complex128: 4096-amplitude tiles (64 KiB LDS), 256 threads x 16 registers.
+  // complex64: 8192-amplitude tiles (64 KiB LDS), 256 threads x 32 registers.
+  if (precision == QM_C128) return FusedShape{12, 8, 4, 3};
+  return FusedShape{13, 8, 5, 4};
+}
+
+void launch_fused_pass(State *s, const FusedShape &shape, const void *d_pass, const void *d_ops,
+                       hipStream_t st) {
+#define QM_LAUNCH(T, l, r)                                                           \
+  if ((s->precision == QM_C128) == std::is_same<T, C128>::value && shape.L == l && \
+      shape.R == r) {                                                                \
+    launch_fused_t<T, l, r>(s, d_pass, d_ops, st);                                   \
+    return;                                                                          \
+  }
+  QM_FUSED_SHAPES(QM_LAUNCH)
+#undef QM_LAUNCH
+  fail(QM_ERR_UNSUPPORTED, "fused tile shape is not compiled");
+}
+
+}  // namespace qm

@@ -11,22 +11,11 @@
 #include <cstring>
 #include <vector>
 
+#include "dev_types.hpp"
 #include "gate_math.hpp"
 #include "qm_internal.hpp"
 
 namespace qm {
-
-struct alignas(16) C128 {
-  double re, im;
-};
-struct alignas(8) C64 {
-  float re, im;
-};
-template <typename T> struct Real;
-template <> struct Real<C128> { using type = double; };
-template <> struct Real<C64> { using type = float; };
-
-constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------
 // Per-gate kernels: one launch per gate, enumerating only the amplitudes the
@@ -142,204 +131,7 @@ void launch_gate(State *s, const DevGate &g, hipStream_t st) {
     launch_gate_t<C64>((C64 *)s->data, s->qubits, g, st);
 }
 
-// ---------------------------------------------------------------------------
-// Fused tile pass.  One workgroup owns one tile of 2^K amplitudes at a time
-// (grid-stride over tiles); each thread keeps 2^R amplitudes in registers.
-// Gates on register bits run in registers; a new register set is reached by
-// an LDS transpose.  LDS is addressed through a 3-bit XOR swizzle so that 8
-// consecutive lanes touching power-of-two strides hit distinct 16-byte bank
-// groups.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t swz(uint32_t j) {
-  uint32_t h = j >> 3;
-  h ^= h >> 3;
-  h ^= h >> 6;
-  h ^= h >> 9;
-  return j ^ (h & 7);
-}
-
-template <typename F, int R, int B, int KIND>
-__device__ __forceinline__ void tile_pair(F *vr, F *vi, const double *cd, uint32_t reg_cmask) {
-  F c[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) c[i] = (F)cd[i];
-#pragma unroll
-  for (int r = 0; r < (1 << R); r++) {
-    if (r & (1 << B)) continue;
-    if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
-    const int r1 = r | (1 << B);
-    if constexpr (KIND == G_MAT) pair_mat<F>(c, vr[r], vi[r], vr[r1], vi[r1]);
-    if constexpr (KIND == G_RX) pair_rx<F>(c[0], c[1], vr[r], vi[r], vr[r1], vi[r1]);
-    if constexpr (KIND == G_RY) pair_ry<F>(c[0], c[1], vr[r], vi[r], vr[r1], vi[r1]);
-    if constexpr (KIND == G_H) pair_h<F>(c[0], vr[r], vi[r], vr[r1], vi[r1]);
-    if constexpr (KIND == G_Y) pair_y<F>(vr[r], vi[r], vr[r1], vi[r1]);
-    if constexpr (KIND == G_X) {
-      F tr = vr[r], ti = vi[r];
-      vr[r] = vr[r1], vi[r] = vi[r1];
-      vr[r1] = tr, vi[r1] = ti;
-    }
-  }
-}
-
-template <typename F, int R, int KIND>
-__device__ __forceinline__ void tile_pair_bit(int b, F *vr, F *vi, const double *c, uint32_t m) {
-  switch (b) {
-    case 0: tile_pair<F, R, 0, KIND>(vr, vi, c, m); break;
-    case 1: tile_pair<F, R, 1, KIND>(vr, vi, c, m); break;
-    case 2: tile_pair<F, R, 2, KIND>(vr, vi, c, m); break;
-    case 3: tile_pair<F, R, 3, KIND>(vr, vi, c, m); break;
-    default:
-      if constexpr (R > 4) tile_pair<F, R, 4, KIND>(vr, vi, c, m);
-      break;
-  }
-}
-
-template <typename F, int R>
-__device__ __forceinline__ void tile_op(const TileOp &op, F *vr, F *vi, uint32_t t, uint64_t base) {
-  if ((base & op.out_cmask) != op.out_cmask) return;
-  if ((t & op.lane_cmask) != op.lane_cmask) return;
-  if (op.rbit >= 0) {
-    switch (op.kind) {
-      case G_MAT: tile_pair_bit<F, R, G_MAT>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
-      case G_RX: tile_pair_bit<F, R, G_RX>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
-      case G_RY: tile_pair_bit<F, R, G_RY>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
-      case G_H: tile_pair_bit<F, R, G_H>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
-      case G_Y: tile_pair_bit<F, R, G_Y>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
-      case G_X: tile_pair_bit<F, R, G_X>(op.rbit, vr, vi, op.c, op.reg_cmask); break;
-      default: break;
-    }
-    return;
-  }
-  // Diagonal: phase D where the target bit is one, A (or nothing for
-  // diag(1, D)) where it is zero.
-  const F ar = (F)op.c[0], ai = (F)op.c[1], dr = (F)op.c[6], di = (F)op.c[7];
-  const bool hi_out = (base & op.dout) != 0;
-  const bool hi_lane = (t & op.dlane) != 0;
-  const bool full = op.kind == G_DIAG;
-#pragma unroll
-  for (int r = 0; r < (1 << R); r++) {
-    if (((uint32_t)r & op.reg_cmask) != op.reg_cmask) continue;
-    const bool hi = hi_out || hi_lane || (((uint32_t)r & op.dreg) != 0);
-    if (full) {
-      mul_phase<F>(hi ? dr : ar, hi ? di : ai, vr[r], vi[r]);
-    } else if (hi) {
-      mul_phase<F>(dr, di, vr[r], vi[r]);
-    }
-  }
-}
-
-template <typename T, int L, int R>
-__global__ __launch_bounds__(1 << L) void k_fused(T *__restrict__ s, const DevPass *__restrict__ pass,
-                                                  const TileOp *__restrict__ prog, uint64_t tiles) {
-  using F = typename Real<T>::type;
-  constexpr int REGS = 1 << R;
-  extern __shared__ char lds_raw[];
-  T *lds = reinterpret_cast<T *>(lds_raw);
-  const DevPass &d = *pass;
-  const uint32_t t = threadIdx.x;
-  uint64_t load_lane = 0, store_lane = 0;
-#pragma unroll
-  for (int k = 0; k < L; k++) {
-    if ((t >> k) & 1) {
-      load_lane |= 1ull << d.load_lane_gbit[k];
-      store_lane |= 1ull << d.store_lane_gbit[k];
-    }
-  }
-  const int nph = d.nphases;
-  const TileOp *ops = prog + d.op_offset;
-  F vr[REGS], vi[REGS];
-
-  auto lane_local = [&](int slot) {
-    uint32_t j = 0;
-#pragma unroll
-    for (int k = 0; k < L; k++)
-      if ((t >> k) & 1) j |= 1u << d.lanepos[slot][k];
-    return j;
-  };
-  auto transpose = [&](int from, int to) {
-    __syncthreads();
-    const uint32_t jf = lane_local(from);
-#pragma unroll
-    for (int r = 0; r < REGS; r++) {
-      T a;
-      a.re = vr[r];
-      a.im = vi[r];
-      lds[swz(jf | d.regloc[from][r])] = a;
-    }
-    __syncthreads();
-    const uint32_t jt = lane_local(to);
-#pragma unroll
-    for (int r = 0; r < REGS; r++) {
-      T a = lds[swz(jt | d.regloc[to][r])];
-      vr[r] = a.re;
-      vi[r] = a.im;
-    }
-  };
-
-  for (uint64_t tau = blockIdx.x; tau < tiles; tau += gridDim.x) {
-    uint64_t base = 0;
-    for (int k = 0; k < d.nruns; k++)
-      base |= ((tau >> d.run_src[k]) & ((1ull << d.run_len[k]) - 1)) << d.run_dst[k];
-    const uint64_t lb = base | load_lane;
-#pragma unroll
-    for (int r = 0; r < REGS; r++) {
-      T a = s[lb | d.load_reg_off[r]];
-      vr[r] = a.re;
-      vi[r] = a.im;
-    }
-    int cur = d.load_direct ? 0 : kIo;
-    for (int ph = 0; ph < nph; ph++) {
-      if (cur != ph) transpose(cur, ph);
-      cur = ph;
-      const TileOp *po = ops + d.phase_op_begin[ph];
-      const int cnt = d.phase_op_count[ph];
-      for (int o = 0; o < cnt; o++) tile_op<F, R>(po[o], vr, vi, t, base);
-    }
-    if (!d.store_direct) transpose(cur, kIo);
-    const uint64_t sb = base | store_lane;
-#pragma unroll
-    for (int r = 0; r < REGS; r++) {
-      T a;
-      a.re = vr[r];
-      a.im = vi[r];
-      s[sb | d.store_reg_off[r]] = a;
-    }
-  }
-}
-
-template <typename T, int L, int R>
-static void launch_fused_t(State *st, const FusedPass &p, const void *d_pass, const void *d_ops,
-                           hipStream_t stream) {
-  const int K = L + R;
-  const uint64_t tiles = 1ull << (st->qubits - K);
-  const size_t lds = sizeof(T) << K;
-  static bool configured = false;
-  if (!configured) {
-    QM_HIP(hipFuncSetAttribute((const void *)k_fused<T, L, R>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    configured = true;
-  }
-  const uint64_t want = (uint64_t)st->ctx->compute_units * 8;
-  const int blocks = (int)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
-  k_fused<T, L, R><<<blocks, 1 << L, lds, stream>>>((T *)st->data, (const DevPass *)d_pass,
-                                                     (const TileOp *)d_ops, tiles);
-  QM_HIP(hipGetLastError());
-}
-
-FusedShape fused_shape(int precision) {
-  // complex128: 4096-amplitude tiles (64 KiB LDS), 256 threads x 16 registers.
-  // complex64: 8192-amplitude tiles (64 KiB LDS), 256 threads x 32 registers.
-  if (precision == QM_C128) return FusedShape{12, 8, 4, 3};
-  return FusedShape{13, 8, 5, 4};
-}
-
-void launch_fused_pass(State *s, const FusedPass &p, const void *d_pass, const void *d_ops,
-                       hipStream_t st) {
-  if (s->precision == QM_C128)
-    launch_fused_t<C128, 8, 4>(s, p, d_pass, d_ops, st);
-  else
-    launch_fused_t<C64, 8, 5>(s, p, d_pass, d_ops, st);
-}
+// The fused tile-pass kernel lives in fused.hip.
 
 // ---------------------------------------------------------------------------
 // Elementwise kernels.

@@ -172,6 +172,7 @@ void fill_mapping(DevPass &d, int slot, uint32_t rmask, int K, int R) {
         b++;
       }
     d.regloc[slot][reg] = (uint16_t)j;
+    d.regswz[slot][reg] = (uint16_t)lds_swizzle(j);
   }
 }
 

@@ -90,6 +90,7 @@ struct DevPass {
   int32_t phase_op_begin[kMaxPhases];  // relative to op_offset
   int32_t phase_op_count[kMaxPhases];
   uint16_t regloc[kMaxPhases + 1][kMaxRegs];  // local index of register r per mapping
+  uint16_t regswz[kMaxPhases + 1][kMaxRegs];  // lds_swizzle(regloc), see lds_swizzle()
   uint8_t lanepos[kMaxPhases + 1][16];        // local position of thread bit k per mapping
   uint8_t load_lane_gbit[16];                 // global bit of thread bit k under the load mapping
   uint8_t store_lane_gbit[16];
@@ -98,6 +99,20 @@ struct DevPass {
   uint8_t run_len[16];
   uint8_t qpos[16];  // tile qubits ascending
 };
+
+// LDS address swizzle: XOR of the low 3 bits of a 16-byte-slot index with a
+// fold of its higher bits.  Linear over GF(2), so the swizzle of a disjoint OR
+// is the XOR of the swizzles (lane part and register part are swizzled apart).
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint32_t lds_swizzle(uint32_t j) {
+  uint32_t h = j >> 3;
+  h ^= h >> 3;
+  h ^= h >> 6;
+  h ^= h >> 9;
+  return j ^ (h & 7);
+}
 
 struct FusedShape {
   int K;           // tile bits

@@ -51,6 +51,8 @@ struct Ctx {
   void *d_prog = nullptr;
   int64_t prog_cap = 0;
   bool disposed = false;
+  // Fused tile shape per precision (set by qm_ctx_set_fused_shape or the autotuner).
+  FusedShape shape[2] = {FusedShape{12, 8, 4, 3}, FusedShape{13, 8, 5, 4}};
 };
 
 struct State {
@@ -73,9 +75,10 @@ void *ctx_program(Ctx *c, int64_t bytes);
 
 // ---- kernel launchers (kernels.hip) ---------------------------------------
 void launch_gate(State *s, const DevGate &g, hipStream_t st);
-void launch_fused_pass(State *s, const FusedPass &p, const void *d_pass, const void *d_ops,
+void launch_fused_pass(State *s, const FusedShape &shape, const void *d_pass, const void *d_ops,
                        hipStream_t st);
-FusedShape fused_shape(int precision);
+FusedShape default_fused_shape(int precision);
+bool fused_shape_supported(int precision, int L, int R);
 void launch_set_basis(State *s, int64_t index, hipStream_t st);  // |index>
 void launch_scale_pairs(State *s, int target, bool jump, double scale, double jump_scale,
                         double stay_scale, hipStream_t st);
