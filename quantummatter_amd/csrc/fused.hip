@@ -18,108 +18,155 @@
 
 namespace qm {
 
+// ---- register-resident gate application -------------------------------------
+// Every op is straight-line code over the registers: the register bit of the
+// target is a template parameter (dispatched once per op), and control bits
+// held in registers select, per pair, either the gate or the identity by a
+// scalar select of coefficients -- both exact, so there are no per-register
+// branches and no register copies around them.
+
 template <typename F, int REGS, int B, int KIND>
-__device__ __forceinline__ void reg_pair(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, F c0, F c1, F c2,
-                                         F c3, F c4, F c5, F c6, F c7) {
+__device__ __forceinline__ void reg_pair(F (&vr)[REGS], F (&vi)[REGS], F c0, F c1) {
 #pragma unroll
   for (int r = 0; r < REGS; r++) {
-    if (r & (1 << B)) continue;
-    if (((uint32_t)r & m) != m) continue;
-    const int r1 = r | (1 << B);
-    if constexpr (KIND == G_MAT) {
-      const F c[8] = {c0, c1, c2, c3, c4, c5, c6, c7};
-      pair_mat<F>(c, vr[r], vi[r], vr[r1], vi[r1]);
-    } else if constexpr (KIND == G_RX) {
-      pair_rx<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
-    } else if constexpr (KIND == G_RY) {
-      pair_ry<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
-    } else if constexpr (KIND == G_H) {
-      pair_h<F>(c0, vr[r], vi[r], vr[r1], vi[r1]);
-    } else if constexpr (KIND == G_Y) {
-      pair_y<F>(vr[r], vi[r], vr[r1], vi[r1]);
-    } else {  // G_X
-      const F tr = vr[r], ti = vi[r];
-      vr[r] = vr[r1];
-      vi[r] = vi[r1];
-      vr[r1] = tr;
-      vi[r1] = ti;
+    if constexpr (true) {
+      if (r & (1 << B)) continue;
+      const int r1 = r | (1 << B);
+      if constexpr (KIND == G_RX) pair_rx<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
+      if constexpr (KIND == G_RY) pair_ry<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
+      if constexpr (KIND == G_H) pair_h<F>(c0, vr[r], vi[r], vr[r1], vi[r1]);
+      if constexpr (KIND == G_Y) pair_y<F>(vr[r], vi[r], vr[r1], vi[r1]);
+      if constexpr (KIND == G_X) {
+        const F tr = vr[r], ti = vi[r];
+        vr[r] = vr[r1];
+        vi[r] = vi[r1];
+        vr[r1] = tr;
+        vi[r1] = ti;
+      }
     }
   }
 }
 
-template <typename F, int REGS, int KIND>
-__device__ __forceinline__ void reg_pair_bit(int b, F (&vr)[REGS], F (&vi)[REGS], uint32_t m, F c0,
-                                             F c1, F c2, F c3, F c4, F c5, F c6, F c7) {
-  switch (b) {
-    case 0: reg_pair<F, REGS, 0, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
-    case 1: reg_pair<F, REGS, 1, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
-    case 2: reg_pair<F, REGS, 2, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
-    case 3: reg_pair<F, REGS, 3, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7); break;
-    default:
-      if constexpr (REGS > 16) reg_pair<F, REGS, 4, KIND>(vr, vi, m, c0, c1, c2, c3, c4, c5, c6, c7);
-      break;
+// General 2x2 (the reference's form) with a register control mask m.
+template <typename F, int REGS, int B>
+__device__ __forceinline__ void reg_mat(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, const F (&c)[8]) {
+#pragma unroll
+  for (int r = 0; r < REGS; r++) {
+    if (r & (1 << B)) continue;
+    const int r1 = r | (1 << B);
+    const bool on = ((uint32_t)r & m) == m;
+    const F k[8] = {on ? c[0] : (F)1, on ? c[1] : (F)0, on ? c[2] : (F)0, on ? c[3] : (F)0,
+                    on ? c[4] : (F)0, on ? c[5] : (F)0, on ? c[6] : (F)1, on ? c[7] : (F)0};
+    pair_mat<F>(k, vr[r], vi[r], vr[r1], vi[r1]);
   }
 }
+
+// Controlled exchange with a register control mask m.
+template <typename F, int REGS, int B>
+__device__ __forceinline__ void reg_cx(F (&vr)[REGS], F (&vi)[REGS], uint32_t m) {
+#pragma unroll
+  for (int r = 0; r < REGS; r++) {
+    if (r & (1 << B)) continue;
+    const int r1 = r | (1 << B);
+    const bool on = ((uint32_t)r & m) == m;
+    const F ar = vr[r], ai = vi[r], br = vr[r1], bi = vi[r1];
+    vr[r] = on ? br : ar;
+    vi[r] = on ? bi : ai;
+    vr[r1] = on ? ar : br;
+    vi[r1] = on ? ai : bi;
+  }
+}
+
+// Diagonal on register bit B: A on the zero half (skipped for diag(1, D)), D on
+// the one half; registers failing the control mask m get the identity.
+template <typename F, int REGS, int B>
+__device__ __forceinline__ void reg_diag(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, bool full, F ar,
+                                         F ai, F dr, F di) {
+#pragma unroll
+  for (int r = 0; r < REGS; r++) {
+    const bool on = ((uint32_t)r & m) == m;
+    if (r & (1 << B)) {
+      mul_phase<F>(on ? dr : (F)1, on ? di : (F)0, vr[r], vi[r]);
+    } else if (full) {
+      mul_phase<F>(on ? ar : (F)1, on ? ai : (F)0, vr[r], vi[r]);
+    }
+  }
+}
+
+// One phase for every register (target on a lane bit or outside the tile).
+template <typename F, int REGS>
+__device__ __forceinline__ void reg_phase_all(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, F pr, F pi) {
+#pragma unroll
+  for (int r = 0; r < REGS; r++) {
+    const bool on = ((uint32_t)r & m) == m;
+    mul_phase<F>(on ? pr : (F)1, on ? pi : (F)0, vr[r], vi[r]);
+  }
+}
+
+#define QM_BIT_SWITCH(b, CALL)             \
+  switch (b) {                             \
+    case 0: { constexpr int BB = 0; CALL; } break; \
+    case 1: { constexpr int BB = 1; CALL; } break; \
+    case 2: { constexpr int BB = 2; CALL; } break; \
+    case 3: { constexpr int BB = 3; CALL; } break; \
+    default: if constexpr (REGS > 16) { constexpr int BB = 4; CALL; } break; \
+  }
 
 template <typename F, int REGS>
 __device__ __forceinline__ void apply_tile_op(const TileOp *__restrict__ op, F (&vr)[REGS],
                                               F (&vi)[REGS], uint32_t t, uint64_t base) {
   const int kind = op->kind;
   const int rbit = op->rbit;
-  const uint32_t reg_cmask = op->reg_cmask;
-  const uint32_t lane_cmask = op->lane_cmask;
-  const uint64_t out_cmask = op->out_cmask;
-  if ((base & out_cmask) != out_cmask) return;
-  if ((t & lane_cmask) != lane_cmask) return;
-  const F c0 = (F)op->c[0], c1 = (F)op->c[1], c6 = (F)op->c[6], c7 = (F)op->c[7];
+  const uint32_t m = op->reg_cmask;
+  if ((base & op->out_cmask) != op->out_cmask) return;
+  if ((t & op->lane_cmask) != op->lane_cmask) return;
   if (rbit >= 0) {
-    switch (kind) {
-      case G_MAT: {
-        const F c2 = (F)op->c[2], c3 = (F)op->c[3], c4 = (F)op->c[4], c5 = (F)op->c[5];
-        reg_pair_bit<F, REGS, G_MAT>(rbit, vr, vi, reg_cmask, c0, c1, c2, c3, c4, c5, c6, c7);
-        break;
+    if (kind == G_X) {
+      if (m == 0) {
+        QM_BIT_SWITCH(rbit, (reg_pair<F, REGS, BB, G_X>(vr, vi, 0, 0)))
+      } else {
+        QM_BIT_SWITCH(rbit, (reg_cx<F, REGS, BB>(vr, vi, m)))
       }
-      case G_RX: reg_pair_bit<F, REGS, G_RX>(rbit, vr, vi, reg_cmask, c0, c1, 0, 0, 0, 0, 0, 0); break;
-      case G_RY: reg_pair_bit<F, REGS, G_RY>(rbit, vr, vi, reg_cmask, c0, c1, 0, 0, 0, 0, 0, 0); break;
-      case G_H: reg_pair_bit<F, REGS, G_H>(rbit, vr, vi, reg_cmask, c0, 0, 0, 0, 0, 0, 0, 0); break;
-      case G_Y: reg_pair_bit<F, REGS, G_Y>(rbit, vr, vi, reg_cmask, 0, 0, 0, 0, 0, 0, 0, 0); break;
-      default: reg_pair_bit<F, REGS, G_X>(rbit, vr, vi, reg_cmask, 0, 0, 0, 0, 0, 0, 0, 0); break;
+      return;
+    }
+    if (kind == G_MAT) {
+      F c[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) c[i] = (F)op->c[i];
+      QM_BIT_SWITCH(rbit, (reg_mat<F, REGS, BB>(vr, vi, m, c)))
+      return;
+    }
+    const F c0 = (F)op->c[0], c1 = (F)op->c[1];
+    switch (kind) {
+      case G_RX: QM_BIT_SWITCH(rbit, (reg_pair<F, REGS, BB, G_RX>(vr, vi, c0, c1))) break;
+      case G_RY: QM_BIT_SWITCH(rbit, (reg_pair<F, REGS, BB, G_RY>(vr, vi, c0, c1))) break;
+      case G_H: QM_BIT_SWITCH(rbit, (reg_pair<F, REGS, BB, G_H>(vr, vi, c0, c1))) break;
+      default: QM_BIT_SWITCH(rbit, (reg_pair<F, REGS, BB, G_Y>(vr, vi, c0, c1))) break;
     }
     return;
   }
-  // Diagonal: D where the target bit is one; A (DIAG) or nothing (DIAG1) where zero.
+  const bool full = kind == G_DIAG;
+  const F ar = (F)op->c[0], ai = (F)op->c[1], dr = (F)op->c[6], di = (F)op->c[7];
   const uint32_t dreg = op->dreg;
-  const bool hi_fixed = ((base & op->dout) != 0) || ((t & op->dlane) != 0);
-  if (kind == G_DIAG) {
-#pragma unroll
-    for (int r = 0; r < REGS; r++) {
-      if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
-      const bool hi = hi_fixed || (((uint32_t)r & dreg) != 0);
-      mul_phase<F>(hi ? c6 : c0, hi ? c7 : c1, vr[r], vi[r]);
-    }
-  } else if (op->dlane == 0) {
-    // The target is a register bit or outside the tile: a uniform choice per register.
-#pragma unroll
-    for (int r = 0; r < REGS; r++) {
-      if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
-      if (hi_fixed || (((uint32_t)r & dreg) != 0)) mul_phase<F>(c6, c7, vr[r], vi[r]);
-    }
-  } else if (hi_fixed) {
-#pragma unroll
-    for (int r = 0; r < REGS; r++) {
-      if (((uint32_t)r & reg_cmask) != reg_cmask) continue;
-      mul_phase<F>(c6, c7, vr[r], vi[r]);
-    }
+  if (dreg != 0) {
+    const int b = __builtin_ctz(dreg);
+    QM_BIT_SWITCH(b, (reg_diag<F, REGS, BB>(vr, vi, m, full, ar, ai, dr, di)))
+    return;
   }
+  // Target on a lane bit (per-lane choice) or outside the tile (uniform choice).
+  const bool hi = ((base & op->dout) != 0) || ((t & op->dlane) != 0);
+  if (!full && op->dlane == 0 && !hi) return;  // diag(1, D) on a zero half: nothing to do
+  const F pr = hi ? dr : (full ? ar : (F)1);
+  const F pi = hi ? di : (full ? ai : (F)0);
+  reg_phase_all<F, REGS>(vr, vi, m, pr, pi);
 }
 
 template <int L>
 __device__ __forceinline__ uint32_t lane_swizzled(const DevPass &d, int slot, uint32_t t) {
   uint32_t j = 0;
 #pragma unroll
-  for (int k = 0; k < L; k++)
-    if ((t >> k) & 1) j |= 1u << d.lanepos[slot][k];
-  return lds_swizzle(j);
+  for (int k = 0; k < L; k++) j ^= ((t >> k) & 1) ? d.lane_swz[slot][k] : 0u;
+  return j;
 }
 
 template <typename T, typename F, int L, int REGS>
@@ -156,10 +203,9 @@ __global__ __launch_bounds__(1 << L, 2) void k_fused(T *__restrict__ s, const De
   uint64_t load_lane = 0, store_lane = 0;
 #pragma unroll
   for (int k = 0; k < L; k++) {
-    if ((t >> k) & 1) {
-      load_lane |= 1ull << d.load_lane_gbit[k];
-      store_lane |= 1ull << d.store_lane_gbit[k];
-    }
+    const bool on = (t >> k) & 1;
+    load_lane |= on ? d.load_lane_off[k] : 0ull;
+    store_lane |= on ? d.store_lane_off[k] : 0ull;
   }
   const int nph = d.nphases;
   const bool load_direct = d.load_direct != 0, store_direct = d.store_direct != 0;

@@ -151,6 +151,29 @@ std::vector<PGate> expand(const std::vector<DevGate> &in) {
   return out;
 }
 
+// The reference's general coefficients (A, B, C, D) of a pair kind
+// (PreparedStateVectorGate.cs:34-65).
+void to_matrix(const DevGate &g, double *m) {
+  for (int i = 0; i < 8; i++) m[i] = 0;
+  switch (g.kind) {
+    case G_RX:  // (c, -i s; -i s, c)
+      m[0] = g.c[0], m[3] = -g.c[1], m[5] = -g.c[1], m[6] = g.c[0];
+      break;
+    case G_RY:  // (c, -s; s, c)
+      m[0] = g.c[0], m[2] = -g.c[1], m[4] = g.c[1], m[6] = g.c[0];
+      break;
+    case G_H:
+      m[0] = m[2] = m[4] = g.c[0], m[6] = -g.c[0];
+      break;
+    case G_Y:  // (0, -i; i, 0)
+      m[3] = -1, m[5] = 1;
+      break;
+    default:
+      for (int i = 0; i < 8; i++) m[i] = g.c[i];
+      break;
+  }
+}
+
 // Rank of bit p among the set bits of m (number of set bits below p).
 inline int rank_in(uint32_t m, int p) { return popc(m & ((1u << p) - 1)); }
 
@@ -160,6 +183,7 @@ void fill_mapping(DevPass &d, int slot, uint32_t rmask, int K, int R) {
     if (rmask & (1u << p)) {
       (void)r;
     } else {
+      d.lane_swz[slot][k] = lds_swizzle(1u << p);
       d.lanepos[slot][k++] = (uint8_t)p;
     }
   }
@@ -325,7 +349,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     d.load_direct = (rsets.front() & low_local) == 0;
     d.store_direct = (rsets.back() & low_local) == 0;
     pass.transposes = (d.nphases - 1) + (d.load_direct ? 0 : 1) + (d.store_direct ? 0 : 1);
-    auto set_io = [&](int slot, uint64_t *reg_off, uint8_t *lane_gbit) {
+    auto set_io = [&](int slot, uint64_t *reg_off, uint8_t *lane_gbit, uint64_t *lane_off) {
       for (int reg = 0; reg < (1 << R); reg++) {
         uint32_t j = d.regloc[slot][reg];
         uint64_t g = 0;
@@ -333,10 +357,14 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           if (j & (1u << p)) g |= (uint64_t)1 << qpos[p];
         reg_off[reg] = g;
       }
-      for (int k = 0; k < L; k++) lane_gbit[k] = (uint8_t)qpos[d.lanepos[slot][k]];
+      for (int k = 0; k < L; k++) {
+        lane_gbit[k] = (uint8_t)qpos[d.lanepos[slot][k]];
+        lane_off[k] = (uint64_t)1 << lane_gbit[k];
+      }
     };
-    set_io(d.load_direct ? 0 : kIo, d.load_reg_off, d.load_lane_gbit);
-    set_io(d.store_direct ? d.nphases - 1 : kIo, d.store_reg_off, d.store_lane_gbit);
+    set_io(d.load_direct ? 0 : kIo, d.load_reg_off, d.load_lane_gbit, d.load_lane_off);
+    set_io(d.store_direct ? d.nphases - 1 : kIo, d.store_reg_off, d.store_lane_gbit,
+           d.store_lane_off);
 
     for (int ph = 0; ph < d.nphases; ph++) {
       uint32_t rset = rsets[ph];
@@ -367,6 +395,12 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           t.dreg = 1u << rank_in(rset, lp);
         } else {
           t.dlane = 1u << rank_in(~rset & ((1u << K) - 1), lp);
+        }
+        if (t.rbit >= 0 && t.reg_cmask != 0 && t.kind != G_X) {
+          // Register-controlled: the kernel runs the general 2x2 form, selecting the
+          // identity for pairs whose control bits are clear (exact either way).
+          to_matrix(x.g, t.c);
+          t.kind = G_MAT;
         }
         pass.ops.push_back(t);
         pass.gates++;

@@ -91,6 +91,9 @@ struct DevPass {
   int32_t phase_op_count[kMaxPhases];
   uint16_t regloc[kMaxPhases + 1][kMaxRegs];  // local index of register r per mapping
   uint16_t regswz[kMaxPhases + 1][kMaxRegs];  // lds_swizzle(regloc), see lds_swizzle()
+  uint32_t lane_swz[kMaxPhases + 1][16];      // lds_swizzle(1 << lanepos) of thread bit k
+  uint64_t load_lane_off[16];                 // global offset of thread bit k (load mapping)
+  uint64_t store_lane_off[16];                // ... (store mapping)
   uint8_t lanepos[kMaxPhases + 1][16];        // local position of thread bit k per mapping
   uint8_t load_lane_gbit[16];                 // global bit of thread bit k under the load mapping
   uint8_t store_lane_gbit[16];
