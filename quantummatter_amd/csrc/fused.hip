@@ -32,9 +32,9 @@ __device__ __forceinline__ void reg_pair(F (&vr)[REGS], F (&vi)[REGS], F c0, F c
     if constexpr (true) {
       if (r & (1 << B)) continue;
       const int r1 = r | (1 << B);
-      if constexpr (KIND == G_RX) pair_rx<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
-      if constexpr (KIND == G_RY) pair_ry<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
-      if constexpr (KIND == G_H) pair_h<F>(c0, vr[r], vi[r], vr[r1], vi[r1]);
+      if constexpr (KIND == G_RX) pair_rx_fma<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
+      if constexpr (KIND == G_RY) pair_ry_fma<F>(c0, c1, vr[r], vi[r], vr[r1], vi[r1]);
+      if constexpr (KIND == G_H) pair_h_fma<F>(c0, vr[r], vi[r], vr[r1], vi[r1]);
       if constexpr (KIND == G_Y) pair_y<F>(vr[r], vi[r], vr[r1], vi[r1]);
       if constexpr (KIND == G_X) {
         const F tr = vr[r], ti = vi[r];
@@ -57,7 +57,7 @@ __device__ __forceinline__ void reg_mat(F (&vr)[REGS], F (&vi)[REGS], uint32_t m
     const bool on = ((uint32_t)r & m) == m;
     const F k[8] = {on ? c[0] : (F)1, on ? c[1] : (F)0, on ? c[2] : (F)0, on ? c[3] : (F)0,
                     on ? c[4] : (F)0, on ? c[5] : (F)0, on ? c[6] : (F)1, on ? c[7] : (F)0};
-    pair_mat<F>(k, vr[r], vi[r], vr[r1], vi[r1]);
+    pair_mat_fma<F>(k, vr[r], vi[r], vr[r1], vi[r1]);
   }
 }
 
@@ -86,9 +86,9 @@ __device__ __forceinline__ void reg_diag(F (&vr)[REGS], F (&vi)[REGS], uint32_t 
   for (int r = 0; r < REGS; r++) {
     const bool on = ((uint32_t)r & m) == m;
     if (r & (1 << B)) {
-      mul_phase<F>(on ? dr : (F)1, on ? di : (F)0, vr[r], vi[r]);
+      mul_phase_fma<F>(on ? dr : (F)1, on ? di : (F)0, vr[r], vi[r]);
     } else if (full) {
-      mul_phase<F>(on ? ar : (F)1, on ? ai : (F)0, vr[r], vi[r]);
+      mul_phase_fma<F>(on ? ar : (F)1, on ? ai : (F)0, vr[r], vi[r]);
     }
   }
 }
@@ -99,7 +99,7 @@ __device__ __forceinline__ void reg_phase_all(F (&vr)[REGS], F (&vi)[REGS], uint
 #pragma unroll
   for (int r = 0; r < REGS; r++) {
     const bool on = ((uint32_t)r & m) == m;
-    mul_phase<F>(on ? pr : (F)1, on ? pi : (F)0, vr[r], vi[r]);
+    mul_phase_fma<F>(on ? pr : (F)1, on ? pi : (F)0, vr[r], vi[r]);
   }
 }
 

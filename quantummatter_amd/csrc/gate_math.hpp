@@ -74,4 +74,62 @@ QM_HD void mul_phase(F pr, F pi, F &xr, F &xi) {
   xi = x1 * pr + x0 * pi;
 }
 
+// ---------------------------------------------------------------------------
+// Fused-multiply-add forms used by the fused tile passes.  Same products and
+// sums as above with one rounding less per fused pair, so the results differ
+// from the reference's rounding by at most an ulp per update (far inside the
+// 1e-12 amplitude tolerance) and cost a third fewer instructions.
+// ---------------------------------------------------------------------------
+template <typename F>
+QM_HD F fma_(F a, F b, F c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+QM_HD float fma_<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
+template <typename F>
+QM_HD void pair_mat_fma(const F *c, F &xr, F &xi, F &yr, F &yi) {
+  const F x0 = xr, x1 = xi, y0 = yr, y1 = yi;
+  xr = fma_(-c[3], y1, fma_(c[2], y0, fma_(-c[1], x1, c[0] * x0)));
+  xi = fma_(c[2], y1, fma_(c[3], y0, fma_(c[0], x1, c[1] * x0)));
+  yr = fma_(-c[7], y1, fma_(c[6], y0, fma_(-c[5], x1, c[4] * x0)));
+  yi = fma_(c[6], y1, fma_(c[7], y0, fma_(c[4], x1, c[5] * x0)));
+}
+
+template <typename F>
+QM_HD void pair_rx_fma(F c, F s, F &xr, F &xi, F &yr, F &yi) {
+  const F x0 = xr, x1 = xi, y0 = yr, y1 = yi;
+  xr = fma_(s, y1, c * x0);
+  xi = fma_(-s, y0, c * x1);
+  yr = fma_(s, x1, c * y0);
+  yi = fma_(-s, x0, c * y1);
+}
+
+template <typename F>
+QM_HD void pair_ry_fma(F c, F s, F &xr, F &xi, F &yr, F &yi) {
+  const F x0 = xr, x1 = xi, y0 = yr, y1 = yi;
+  xr = fma_(-s, y0, c * x0);
+  xi = fma_(-s, y1, c * x1);
+  yr = fma_(s, x0, c * y0);
+  yi = fma_(s, x1, c * y1);
+}
+
+template <typename F>
+QM_HD void pair_h_fma(F h, F &xr, F &xi, F &yr, F &yi) {
+  const F x0 = xr, x1 = xi, y0 = yr, y1 = yi;
+  xr = fma_(h, y0, h * x0);
+  xi = fma_(h, y1, h * x1);
+  yr = fma_(-h, y0, h * x0);
+  yi = fma_(-h, y1, h * x1);
+}
+
+template <typename F>
+QM_HD void mul_phase_fma(F pr, F pi, F &xr, F &xi) {
+  const F x0 = xr, x1 = xi;
+  xr = fma_(-x1, pi, x0 * pr);
+  xi = fma_(x0, pi, x1 * pr);
+}
+
 }  // namespace qm

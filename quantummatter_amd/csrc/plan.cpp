@@ -431,6 +431,20 @@ inline void apply_pair_host(int kind, const double *c, double &xr, double &xi, d
   }
 }
 
+// The fused kernel's arithmetic (fused multiply-add forms).
+inline void apply_pair_fused(int kind, const double *c, double &xr, double &xi, double &yr,
+                             double &yi) {
+  switch (kind) {
+    case G_MAT: pair_mat_fma(c, xr, xi, yr, yi); break;
+    case G_RX: pair_rx_fma(c[0], c[1], xr, xi, yr, yi); break;
+    case G_RY: pair_ry_fma(c[0], c[1], xr, xi, yr, yi); break;
+    case G_H: pair_h_fma(c[0], xr, xi, yr, yi); break;
+    case G_Y: pair_y(xr, xi, yr, yi); break;
+    case G_X: std::swap(xr, yr), std::swap(xi, yi); break;
+    default: break;
+  }
+}
+
 }  // namespace
 
 void apply_gate_host(const DevGate &g, double *s, int n) {
@@ -520,7 +534,7 @@ void apply_plan_host(const FusedPlan &plan, double *s, int n) {
                 if (r & (1 << op.rbit)) continue;
                 if ((r & op.reg_cmask) != op.reg_cmask) continue;
                 int r1 = r | (1 << op.rbit);
-                apply_pair_host(op.kind, op.c, vt[2 * r], vt[2 * r + 1], vt[2 * r1], vt[2 * r1 + 1]);
+                apply_pair_fused(op.kind, op.c, vt[2 * r], vt[2 * r + 1], vt[2 * r1], vt[2 * r1 + 1]);
               }
             } else {
               bool hi_fixed = ((t & op.dlane) != 0) || ((base & op.dout) != 0);
@@ -528,9 +542,9 @@ void apply_plan_host(const FusedPlan &plan, double *s, int n) {
                 if ((r & op.reg_cmask) != op.reg_cmask) continue;
                 bool hi = hi_fixed || ((r & op.dreg) != 0);
                 if (hi)
-                  mul_phase(op.c[6], op.c[7], vt[2 * r], vt[2 * r + 1]);
+                  mul_phase_fma(op.c[6], op.c[7], vt[2 * r], vt[2 * r + 1]);
                 else if (op.kind == G_DIAG)
-                  mul_phase(op.c[0], op.c[1], vt[2 * r], vt[2 * r + 1]);
+                  mul_phase_fma(op.c[0], op.c[1], vt[2 * r], vt[2 * r + 1]);
               }
             }
           }

@@ -79,16 +79,24 @@ def test_fused_plan_layer_circuit(n):
         assert np.max(np.abs(got - exp)) <= 1e-13
 
 
-def test_single_gates_are_bit_exact_in_one_tile():
+def test_single_gates_in_one_tile():
+    """Permutations are exact; arithmetic gates differ from the reference's
+    rounding by at most the one rounding a fused multiply-add saves."""
     n = 12
     psi = np.exp(1j * np.arange(1 << n) * 0.37) / math.sqrt(1 << n)
-    for build in (lambda c: c.Rx(3, 0.7), lambda c: c.Ry(11, -2.1), lambda c: c.Rz(0, 0.3),
-                  lambda c: c.H(7), lambda c: c.CNot(2, 9), lambda c: c.Y(4), lambda c: c.T(10),
-                  lambda c: c.Swap(1, 8), lambda c: c.Toffoli(0, 5, 11)):
+    for build, exact in ((lambda c: c.Rx(3, 0.7), False), (lambda c: c.Ry(11, -2.1), False),
+                         (lambda c: c.Rz(0, 0.3), False), (lambda c: c.H(7), False),
+                         (lambda c: c.T(10), False), (lambda c: c.CNot(2, 9), True),
+                         (lambda c: c.Y(4), True), (lambda c: c.Swap(1, 8), True),
+                         (lambda c: c.Toffoli(0, 5, 11), True)):
         c = QuantumCircuit(n)
         build(c)
         ops = c.to_ops()
-        assert np.array_equal(plan_apply_host(n, ops, psi, 12, 8, 3), oracle.execute(n, ops, psi))
+        got, want = plan_apply_host(n, ops, psi, 12, 8, 3), oracle.execute(n, ops, psi)
+        if exact:
+            assert np.array_equal(got, want)
+        else:
+            assert np.max(np.abs(got - want)) <= 2.0 ** -52 * np.max(np.abs(psi))
 
 
 def test_schedule_shape_for_layer_circuits():
