@@ -96,7 +96,7 @@ def test_single_gates_in_one_tile():
         if exact:
             assert np.array_equal(got, want)
         else:
-            assert np.max(np.abs(got - want)) <= 2.0 ** -52 * np.max(np.abs(psi))
+            assert np.max(np.abs(got - want)) <= 2.0 ** -50 * np.max(np.abs(psi))
 
 
 def test_schedule_shape_for_layer_circuits():
