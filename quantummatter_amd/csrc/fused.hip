@@ -79,15 +79,15 @@ __device__ __forceinline__ void reg_cx(F (&vr)[REGS], F (&vi)[REGS], uint32_t m)
 
 // Diagonal on register bit B: A on the zero half (skipped for diag(1, D)), D on
 // the one half; registers failing the control mask m get the identity.
-template <typename F, int REGS, int B>
-__device__ __forceinline__ void reg_diag(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, bool full, F ar,
-                                         F ai, F dr, F di) {
+template <typename F, int REGS, int B, bool FULL>
+__device__ __forceinline__ void reg_diag(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, F ar, F ai, F dr,
+                                         F di) {
 #pragma unroll
   for (int r = 0; r < REGS; r++) {
     const bool on = ((uint32_t)r & m) == m;
     if (r & (1 << B)) {
       mul_phase_fma<F>(on ? dr : (F)1, on ? di : (F)0, vr[r], vi[r]);
-    } else if (full) {
+    } else if constexpr (FULL) {
       mul_phase_fma<F>(on ? ar : (F)1, on ? ai : (F)0, vr[r], vi[r]);
     }
   }
@@ -96,6 +96,11 @@ __device__ __forceinline__ void reg_diag(F (&vr)[REGS], F (&vi)[REGS], uint32_t 
 // One phase for every register (target on a lane bit or outside the tile).
 template <typename F, int REGS>
 __device__ __forceinline__ void reg_phase_all(F (&vr)[REGS], F (&vi)[REGS], uint32_t m, F pr, F pi) {
+  if (m == 0) {
+#pragma unroll
+    for (int r = 0; r < REGS; r++) mul_phase_fma<F>(pr, pi, vr[r], vi[r]);
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < REGS; r++) {
     const bool on = ((uint32_t)r & m) == m;
@@ -113,13 +118,13 @@ __device__ __forceinline__ void reg_phase_all(F (&vr)[REGS], F (&vi)[REGS], uint
   }
 
 template <typename F, int REGS>
-__device__ __forceinline__ void apply_tile_op(const TileOp *__restrict__ op, F (&vr)[REGS],
-                                              F (&vi)[REGS], uint32_t t, uint64_t base) {
+__device__ __forceinline__ int apply_tile_op(const TileOp *__restrict__ op, F (&vr)[REGS],
+                                             F (&vi)[REGS], uint32_t t, uint64_t base) {
   const int kind = op->kind;
   const int rbit = op->rbit;
   const uint32_t m = op->reg_cmask;
-  if ((base & op->out_cmask) != op->out_cmask) return;
-  if ((t & op->lane_cmask) != op->lane_cmask) return;
+  if ((base & op->out_cmask) != op->out_cmask) return 0;
+  if ((t & op->lane_cmask) != op->lane_cmask) return 0;
   if (rbit >= 0) {
     if (kind == G_X) {
       if (m == 0) {
@@ -127,14 +132,14 @@ __device__ __forceinline__ void apply_tile_op(const TileOp *__restrict__ op, F (
       } else {
         QM_BIT_SWITCH(rbit, (reg_cx<F, REGS, BB>(vr, vi, m)))
       }
-      return;
+      return 0;
     }
     if (kind == G_MAT) {
       F c[8];
 #pragma unroll
       for (int i = 0; i < 8; i++) c[i] = (F)op->c[i];
       QM_BIT_SWITCH(rbit, (reg_mat<F, REGS, BB>(vr, vi, m, c)))
-      return;
+      return 0;
     }
     const F c0 = (F)op->c[0], c1 = (F)op->c[1];
     switch (kind) {
@@ -143,22 +148,27 @@ __device__ __forceinline__ void apply_tile_op(const TileOp *__restrict__ op, F (
       case G_H: QM_BIT_SWITCH(rbit, (reg_pair<F, REGS, BB, G_H>(vr, vi, c0, c1))) break;
       default: QM_BIT_SWITCH(rbit, (reg_pair<F, REGS, BB, G_Y>(vr, vi, c0, c1))) break;
     }
-    return;
+    return 0;
   }
   const bool full = kind == G_DIAG;
   const F ar = (F)op->c[0], ai = (F)op->c[1], dr = (F)op->c[6], di = (F)op->c[7];
   const uint32_t dreg = op->dreg;
   if (dreg != 0) {
     const int b = __builtin_ctz(dreg);
-    QM_BIT_SWITCH(b, (reg_diag<F, REGS, BB>(vr, vi, m, full, ar, ai, dr, di)))
-    return;
+    if (full) {
+      QM_BIT_SWITCH(b, (reg_diag<F, REGS, BB, true>(vr, vi, m, ar, ai, dr, di)))
+    } else {
+      QM_BIT_SWITCH(b, (reg_diag<F, REGS, BB, false>(vr, vi, m, ar, ai, dr, di)))
+    }
+    return 0;
   }
   // Target on a lane bit (per-lane choice) or outside the tile (uniform choice).
   const bool hi = ((base & op->dout) != 0) || ((t & op->dlane) != 0);
-  if (!full && op->dlane == 0 && !hi) return;  // diag(1, D) on a zero half: nothing to do
+  if (!full && op->dlane == 0 && !hi) return 0;  // diag(1, D) on a zero half: nothing to do
   const F pr = hi ? dr : (full ? ar : (F)1);
   const F pi = hi ? di : (full ? ai : (F)0);
   reg_phase_all<F, REGS>(vr, vi, m, pr, pi);
+  return 0;
 }
 
 template <int L>
@@ -230,6 +240,14 @@ __global__ __launch_bounds__(1 << L, 2) void k_fused(T *__restrict__ s, const De
       const TileOp *__restrict__ po = ops + d.phase_op_begin[ph];
       const int cnt = d.phase_op_count[ph];
       for (int o = 0; o < cnt; o++) apply_tile_op<F, REGS>(po + o, vr, vi, t, base);
+      const int dcnt = d.phase_diag_count[ph];
+      if (dcnt) {
+        // Merged lane/outside diagonals: one per-lane product, one multiplication.
+        F pr, pi;
+        phase_product<F>(ops + d.phase_diag_begin[ph], dcnt, t, base, pr, pi);
+#pragma unroll
+        for (int r = 0; r < REGS; r++) mul_phase_fma<F>(pr, pi, vr[r], vi[r]);
+      }
     }
     if (!store_direct) lds_transpose<T, F, L, REGS>(lds, d, cur, kIo, t, vr, vi);
     T *dst = s + (base | store_lane);

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <stdexcept>
 
 #include "gate_math.hpp"
@@ -14,6 +15,32 @@ struct Status;
 [[noreturn]] void fail(int code, const std::string &msg);
 
 static inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+uint32_t lds_swizzle_mode(uint32_t j, int mode) {
+  switch (mode) {
+    case 0:
+      return j;
+    case 1: {
+      uint32_t h = j >> 3;
+      h ^= h >> 3;
+      h ^= h >> 6;
+      h ^= h >> 9;
+      return j ^ (h & 7);
+    }
+    case 2:
+      return j ^ ((j >> 3) & 7);
+    default:
+      return j ^ (((j >> 3) ^ (j >> 6) ^ (j >> 9) ^ (j >> 12)) & 7);
+  }
+}
+
+int lds_swizzle_default() {
+  static const int mode = [] {
+    const char *e = std::getenv("QM_LDS_SWIZZLE");  // tuning knob
+    return e ? std::atoi(e) : 3;
+  }();
+  return mode;
+}
 
 void validate_op(const qm_op &op, int n) {
   // QuantumCircuit.Add (QuantumCircuit.cs:97-126): range, distinctness, finite angle.
@@ -174,6 +201,40 @@ void to_matrix(const DevGate &g, double *m) {
   }
 }
 
+// Moves the diagonal ops of one phase whose target and controls are lane bits
+// or outside the tile to the end of the phase, behind a G_PHASEPROD header, when
+// no later pair op of the phase targets any qubit they touch (a diagonal gate
+// commutes with every gate that does not change its qubits).  The kernel then
+// forms their per-lane product once instead of touching every register per op.
+void merge_phase_diagonals(std::vector<TileOp> &ops, const std::vector<uint64_t> &touch,
+                           const std::vector<uint64_t> &pair_target, int begin,
+                           std::vector<TileOp> &tail, std::vector<int> &headers) {
+  const int n = (int)ops.size() - begin;
+  if (n < 2) return;
+  uint64_t later_targets = 0;
+  std::vector<char> move(n, 0);
+  for (int i = n - 1; i >= 0; i--) {
+    const TileOp &t = ops[begin + i];
+    const bool diag = (t.kind == G_DIAG || t.kind == G_DIAG1) && t.rbit < 0;
+    if (diag && t.dreg == 0 && t.reg_cmask == 0 && !(touch[i] & later_targets)) move[i] = 1;
+    later_targets |= pair_target[i];
+  }
+  int count = 0;
+  for (int i = 0; i < n; i++) count += move[i];
+  if (count < 2) return;
+  std::vector<TileOp> kept;
+  headers.push_back((int)tail.size());  // this phase's first merged op in the tail
+  headers.push_back(count);
+  for (int i = 0; i < n; i++) {
+    if (move[i])
+      tail.push_back(ops[begin + i]);
+    else
+      kept.push_back(ops[begin + i]);
+  }
+  ops.resize(begin);
+  ops.insert(ops.end(), kept.begin(), kept.end());
+}
+
 // Rank of bit p among the set bits of m (number of set bits below p).
 inline int rank_in(uint32_t m, int p) { return popc(m & ((1u << p) - 1)); }
 
@@ -227,32 +288,47 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
   for (size_t i = 0; i < G.size(); i++) pending[i] = (int)i;
 
   while (!pending.empty()) {
-    // 1. Tile qubits: greedy scan with dependency blocking.
-    uint64_t Q = low, blocked = 0;
+    // 1. Tile qubits: greedy scan with dependency blocking; spare tile slots
+    //    go to the targets of the deferred gates, then the scan is repeated.
+    uint64_t Q = low;
     std::vector<int> included, deferred;
-    for (size_t k = 0; k < pending.size(); k++) {
-      int idx = pending[k];
-      const PGate &x = G[idx];
-      if (blocked == all) {
-        deferred.insert(deferred.end(), pending.begin() + k, pending.end());
-        break;
-      }
-      if (x.touch & blocked) {
-        deferred.push_back(idx);
-        blocked |= x.touch;
-        continue;
-      }
-      uint64_t tb = (uint64_t)1 << x.g.target;
-      if (x.pair && !(Q & tb)) {
-        if (popc(Q) < K) {
-          Q |= tb;
-        } else {
+    auto scan_tile = [&](uint64_t &q, bool grow) {
+      uint64_t blocked = 0;
+      included.clear();
+      deferred.clear();
+      for (size_t k = 0; k < pending.size(); k++) {
+        int idx = pending[k];
+        const PGate &x = G[idx];
+        if (blocked == all) {
+          deferred.insert(deferred.end(), pending.begin() + k, pending.end());
+          break;
+        }
+        if (x.touch & blocked) {
           deferred.push_back(idx);
           blocked |= x.touch;
           continue;
         }
+        uint64_t tb = (uint64_t)1 << x.g.target;
+        if (x.pair && !(q & tb)) {
+          if (grow && popc(q) < K) {
+            q |= tb;
+          } else {
+            deferred.push_back(idx);
+            blocked |= x.touch;
+            continue;
+          }
+        }
+        included.push_back(idx);
       }
-      included.push_back(idx);
+    };
+    scan_tile(Q, true);
+    if (popc(Q) < K && !deferred.empty()) {
+      for (int idx : deferred) {
+        if (popc(Q) >= K) break;
+        if (G[idx].pair) Q |= (uint64_t)1 << G[idx].g.target;
+      }
+      for (int q = 0; popc(Q) < K && q < n; q++) Q |= (uint64_t)1 << q;
+      scan_tile(Q, false);
     }
     for (int q = 0; popc(Q) < K && q < n; q++) Q |= (uint64_t)1 << q;
 
@@ -302,28 +378,42 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         break;
       }
       uint32_t rset = 0;
-      uint64_t pblocked = 0;
       std::vector<int> here, next;
-      for (int idx : remaining) {
-        const PGate &x = G[idx];
-        if (x.touch & pblocked) {
-          next.push_back(idx);
-          pblocked |= x.touch;
-          continue;
-        }
-        if (x.pair) {
-          uint32_t lb = 1u << local_of[x.g.target];
-          if (!(rset & lb)) {
-            if (popc(rset) < R) {
-              rset |= lb;
-            } else {
-              next.push_back(idx);
-              pblocked |= x.touch;
-              continue;
+      auto scan_phase = [&](uint32_t &rs, bool grow) {
+        uint64_t pblocked = 0;
+        here.clear();
+        next.clear();
+        for (int idx : remaining) {
+          const PGate &x = G[idx];
+          if (x.touch & pblocked) {
+            next.push_back(idx);
+            pblocked |= x.touch;
+            continue;
+          }
+          if (x.pair) {
+            uint32_t lb = 1u << local_of[x.g.target];
+            if (!(rs & lb)) {
+              if (grow && popc(rs) < R) {
+                rs |= lb;
+              } else {
+                next.push_back(idx);
+                pblocked |= x.touch;
+                continue;
+              }
             }
           }
+          here.push_back(idx);
         }
-        here.push_back(idx);
+      };
+      scan_phase(rset, true);
+      if (popc(rset) < R && !next.empty()) {
+        // Spare register bits go to the targets of the deferred gates (in
+        // order), and the phase is rescanned: fewer phases, fewer transposes.
+        for (int idx : next) {
+          if (popc(rset) >= R) break;
+          if (G[idx].pair) rset |= 1u << local_of[G[idx].g.target];
+        }
+        scan_phase(rset, false);
       }
       // Fill the register set from the top local positions (keeps the
       // contiguous low bits on lanes, so loads and stores stay coalesced).
@@ -366,6 +456,9 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     set_io(d.store_direct ? d.nphases - 1 : kIo, d.store_reg_off, d.store_lane_gbit,
            d.store_lane_off);
 
+    std::vector<uint64_t> op_touch, op_pair_target;
+    std::vector<TileOp> tail;  // merged diagonal ops, kept after every phase's ops
+    std::vector<int> headers;
     for (int ph = 0; ph < d.nphases; ph++) {
       uint32_t rset = rsets[ph];
       d.phase_op_begin[ph] = (int32_t)pass.ops.size();
@@ -403,10 +496,25 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           t.kind = G_MAT;
         }
         pass.ops.push_back(t);
+        op_touch.push_back(x.touch);
+        op_pair_target.push_back(x.pair ? ((uint64_t)1 << x.g.target) : 0);
         pass.gates++;
       }
+      headers.clear();
+      merge_phase_diagonals(pass.ops, op_touch, op_pair_target, d.phase_op_begin[ph], tail,
+                            headers);
+      if (!headers.empty()) {
+        d.phase_diag_begin[ph] = headers[0];  // rebased below
+        d.phase_diag_count[ph] = headers[1];
+      }
+      op_touch.clear();
+      op_pair_target.clear();
       d.phase_op_count[ph] = (int32_t)pass.ops.size() - d.phase_op_begin[ph];
     }
+    const int32_t tail_base = (int32_t)pass.ops.size();
+    for (int ph = 0; ph < d.nphases; ph++)
+      if (d.phase_diag_count[ph]) d.phase_diag_begin[ph] += tail_base;
+    pass.ops.insert(pass.ops.end(), tail.begin(), tail.end());
     plan.passes.push_back(std::move(pass));
     pending.swap(deferred);
   }
@@ -547,6 +655,15 @@ void apply_plan_host(const FusedPlan &plan, double *s, int n) {
                   mul_phase_fma(op.c[0], op.c[1], vt[2 * r], vt[2 * r + 1]);
               }
             }
+          }
+        }
+        if (d.phase_diag_count[ph]) {
+          const TileOp *q = &p.ops[d.phase_diag_begin[ph]];
+          for (int t = 0; t < threads; t++) {
+            double pr, pi;
+            phase_product(q, d.phase_diag_count[ph], (uint32_t)t, base, pr, pi);
+            double *vt = &v[2 * (size_t)t * regs];
+            for (int r = 0; r < regs; r++) mul_phase_fma(pr, pi, vt[2 * r], vt[2 * r + 1]);
           }
         }
       }

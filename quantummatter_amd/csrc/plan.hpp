@@ -24,6 +24,10 @@ enum GKind : int32_t {
   G_DIAG = 6,   // diag(A, D) on both halves: A = c[0..1], D = c[6..7]
   G_DIAG1 = 7,  // diag(1, D): A is exactly 1, only the one-half changes
   G_SWAP = 8,   // exchange of (a=1,t=0) with (a=0,t=1); swap_other = a
+  // Fused passes only: the next `rbit` ops are diagonal gates whose targets
+  // and controls are lane bits or outside the tile; their per-lane product is
+  // formed once and applied to every register in one multiplication.
+  G_PHASEPROD = 9,
 };
 
 struct DevGate {
@@ -89,6 +93,10 @@ struct DevPass {
   int32_t op_offset;     // first TileOp of this pass in the program
   int32_t phase_op_begin[kMaxPhases];  // relative to op_offset
   int32_t phase_op_count[kMaxPhases];
+  // Merged lane/outside diagonal ops applied at the end of each phase as one
+  // per-lane phase product (indices relative to op_offset; count 0 = none).
+  int32_t phase_diag_begin[kMaxPhases];
+  int32_t phase_diag_count[kMaxPhases];
   uint16_t regloc[kMaxPhases + 1][kMaxRegs];  // local index of register r per mapping
   uint16_t regswz[kMaxPhases + 1][kMaxRegs];  // lds_swizzle(regloc), see lds_swizzle()
   uint32_t lane_swz[kMaxPhases + 1][16];      // lds_swizzle(1 << lanepos) of thread bit k
@@ -106,15 +114,36 @@ struct DevPass {
 // LDS address swizzle: XOR of the low 3 bits of a 16-byte-slot index with a
 // fold of its higher bits.  Linear over GF(2), so the swizzle of a disjoint OR
 // is the XOR of the swizzles (lane part and register part are swizzled apart).
+// The planner precomputes swizzled addresses into the pass tables, so the
+// swizzle can be chosen per plan without touching the kernel: any map that is
+// linear over GF(2) works.  Mode (default 3): 0 none; 1 cascaded fold;
+// 2 XOR with bits 3..5; 3 XOR with the 3-bit groups 3..5, 6..8, 9..11, 12..14.
+uint32_t lds_swizzle_mode(uint32_t j, int mode);
+int lds_swizzle_default();
+inline uint32_t lds_swizzle(uint32_t j) { return lds_swizzle_mode(j, lds_swizzle_default()); }
+
+// Per-lane product of the diagonal ops q[0..count) that follow a G_PHASEPROD
+// header (lane/outside targets and controls only), shared by the kernel and
+// the host reference.
+template <typename F>
 #if defined(__HIPCC__)
 __host__ __device__
 #endif
-inline uint32_t lds_swizzle(uint32_t j) {
-  uint32_t h = j >> 3;
-  h ^= h >> 3;
-  h ^= h >> 6;
-  h ^= h >> 9;
-  return j ^ (h & 7);
+inline void phase_product(const TileOp *q, int count, uint32_t t, uint64_t base, F &pr, F &pi) {
+  pr = 1;
+  pi = 0;
+  for (int k = 0; k < count; k++) {
+    const bool on = ((base & q[k].out_cmask) == q[k].out_cmask) &&
+                    ((t & q[k].lane_cmask) == q[k].lane_cmask);
+    const bool hi = ((base & q[k].dout) != 0) || ((t & q[k].dlane) != 0);
+    const bool full = q[k].kind == 6;  // G_DIAG
+    const F fr = on ? (F)(hi ? q[k].c[6] : (full ? q[k].c[0] : 1.0)) : (F)1;
+    const F fi = on ? (F)(hi ? q[k].c[7] : (full ? q[k].c[1] : 0.0)) : (F)0;
+    const F nr = pr * fr - pi * fi;
+    const F ni = pr * fi + pi * fr;
+    pr = nr;
+    pi = ni;
+  }
 }
 
 struct FusedShape {
