@@ -215,6 +215,57 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
  * summation order.  Returns the count accumulated on this state. */
 int qm_state_boundary_fallbacks(qm_state *state, int64_t *count);
 
+/* ---------------------------------------------------------------------------
+ * Batched execution: one program over many same-width states, one kernel
+ * launch, no host round trip between the first gate and the last result.
+ * Each state is owned by one workgroup, so per-state decisions (noise
+ * branches, damping masses, CDF searches) stay on the device.  Widths below
+ * qm_batch_min_qubits() are not batched (QM_ERR_UNSUPPORTED).
+ * ------------------------------------------------------------------------- */
+int qm_batch_min_qubits(int32_t precision, int32_t *qubits);
+
+/* NoisyStateVectorSimulator.Run (src/QuantumMatter.Application/Noise/
+ * NoisyStateVectorSimulator.cs:36-62) over `trajectories` independent
+ * trajectories of NoisyCircuit.Execute (src/QuantumMatter.Core/Noise/
+ * NoisyCircuit.cs:48-65) with CpuNoiseChannelBackend.Apply's branch rules
+ * (src/QuantumMatter.Infrastructure/CpuNoiseChannelBackend.cs:10-153).
+ * Trajectory t starts from |0...0> and draws from its own pool
+ * pools[t * pool_stride ...] in step order; after the circuit it takes `shots`
+ * terminal samples with StateVectorSimulator.Run's CDF rule
+ * (src/QuantumMatter.Application/StateVectorSimulator.cs:44-90) from
+ * shot_uniforms[t * shots ...] into samples[t * shots ...], then, when
+ * measure_all, one MeasureAll draw from its pool (StateVector.cs:106-153) into
+ * outcomes[t].  consumed[t] = draws taken from its pool.  Damping and CDF
+ * decisions within the rounding margin are re-decided in the reference's
+ * sequential order; *fallbacks counts them.  Errors: QM_ERR_POOL when a pool
+ * runs out, QM_ERR_MASS for a state without finite positive mass. */
+int qm_trajectories_run(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_step *steps,
+                        int64_t n_steps, const qm_op *ops, int64_t n_ops, int64_t trajectories,
+                        const double *pools, int64_t pool_stride, int64_t shots,
+                        const double *shot_uniforms, int32_t measure_all, int64_t *samples,
+                        int64_t *outcomes, int64_t *consumed, int64_t *fallbacks,
+                        qm_profile *profile);
+
+/* VariationalEnergyEvaluator.EvaluateWithGradient (src/QuantumMatter.Application/
+ * Variational/VariationalEnergyEvaluator.cs:34-86): the energy of the bound
+ * circuit and the parameter-shift gradient, all 1 + 2R circuits (R = rotation
+ * occurrences, src/QuantumMatter.Core/Variational/ParameterizedCircuit.cs:68-96) in one launch.  ops/param_index
+ * are the template (param_index[i] >= 0 for a parameter rotation, which must
+ * be Rx, Ry or Rz).  Expectations use FP64 reductions for both precisions. */
+int qm_energy_gradient_batch(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_op *ops,
+                             const int32_t *param_index, int64_t n_ops, const double *parameters,
+                             int64_t n_params, const qm_pauli *terms, const double *coeffs,
+                             int64_t n_terms, double *energy, double *gradient,
+                             qm_profile *profile);
+
+/* VariationalEnergyEvaluator.Evaluate (VariationalEnergyEvaluator.cs:23-32) for
+ * n_sets parameter vectors (parameter_sets[s * n_params ...]) in one launch;
+ * energies[s] as StateVector.ExpectationEnergy (StateVector.Energy.cs:10-46). */
+int qm_energy_batch(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_op *ops,
+                    const int32_t *param_index, int64_t n_ops, const double *parameter_sets,
+                    int64_t n_params, int64_t n_sets, const qm_pauli *terms, const double *coeffs,
+                    int64_t n_terms, double *energies, qm_profile *profile);
+
 #ifdef __cplusplus
 }
 #endif

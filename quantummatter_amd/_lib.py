@@ -66,6 +66,10 @@ SIGNATURES = {
     "qm_state_boundary_fallbacks": [P, P],
     "qm_plan_fused": [I32, I32, P, I64, P, P, P],
     "qm_plan_apply_host": [I32, P, I64, I32, I32, I32, P],
+    "qm_batch_min_qubits": [I32, P],
+    "qm_trajectories_run": [P, I32, I32, P, I64, P, I64, I64, P, I64, I64, P, I32, P, P, P, P, P],
+    "qm_energy_gradient_batch": [P, I32, I32, P, P, I64, P, I64, P, P, I64, P, P, P],
+    "qm_energy_batch": [P, I32, I32, P, P, I64, P, I64, I64, P, P, I64, P, P],
 }
 _RESTYPE = {"qm_last_error": C.c_char_p, "qm_abi_version": C.c_int32}
 

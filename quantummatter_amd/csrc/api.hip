@@ -30,27 +30,6 @@ void hip_check(hipError_t e, const char *what) {
   }
 }
 
-template <typename Fn>
-static int guarded(Fn &&fn) {
-  try {
-    clear_error();
-    fn();
-    return QM_OK;
-  } catch (const Status &s) {
-    set_error(s.msg);
-    return s.code;
-  } catch (const std::bad_alloc &) {
-    set_error("host allocation failed");
-    return QM_ERR_BUDGET;
-  } catch (const std::exception &e) {
-    set_error(e.what());
-    return QM_ERR_DEVICE;
-  } catch (...) {
-    set_error("unknown failure");
-    return QM_ERR_DEVICE;
-  }
-}
-
 double *ctx_partials(Ctx *c, int64_t n) {
   if (n > c->partial_cap) {
     if (c->d_partial) {
@@ -107,17 +86,17 @@ void *ctx_program(Ctx *c, int64_t bytes) {
   return c->d_prog;
 }
 
-static void use_device(Ctx *c) {
+void use_device(Ctx *c) {
   if (c == nullptr) fail(QM_ERR_ARG, "context is null");
   if (c->disposed) fail(QM_ERR_DISPOSED, "The HIP backend has been disposed.");
   QM_HIP(hipSetDevice(c->device));
 }
 
-static void check_cancel(const volatile int32_t *cancel) {
+void check_cancel(const volatile int32_t *cancel) {
   if (cancel && *cancel) fail(QM_ERR_CANCELLED, "The operation was canceled.");
 }
 
-static std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
+std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
   if (n_ops < 0 || (n_ops > 0 && ops == nullptr)) fail(QM_ERR_ARG, "operations are null");
   std::vector<DevGate> gates;
   gates.reserve((size_t)n_ops);

@@ -1,0 +1,539 @@
+// C ABI of the batched entry points of libqmhip.so: noisy trajectories and
+// parameter-shift / multi-parameter energy evaluation, each one kernel launch
+// over many states (batch.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "batch.hpp"
+#include "qm_internal.hpp"
+
+namespace qm {
+namespace {
+
+// One device allocation for everything a batched call uploads and reads back.
+struct Arena {
+  size_t size = 0;
+  void *base = nullptr;
+  size_t add(size_t bytes) {
+    const size_t off = (size + 255) & ~(size_t)255;
+    size = off + bytes;
+    return off;
+  }
+  template <typename P>
+  P *at(size_t off) const {
+    return reinterpret_cast<P *>(static_cast<char *>(base) + off);
+  }
+  ~Arena() {
+    if (base) (void)hipFree(base);
+  }
+};
+
+struct Groups {
+  std::vector<PauliGroup> groups;
+  std::vector<int64_t> order;  // sorted position k -> original term index
+};
+
+// Terms grouped by flip mask (stable in term order), at most kTermsPerGroup each.
+Groups group_terms(const qm_pauli *terms, int64_t n_terms) {
+  Groups g;
+  g.order.resize((size_t)n_terms);
+  std::iota(g.order.begin(), g.order.end(), 0);
+  std::stable_sort(g.order.begin(), g.order.end(), [&](int64_t a, int64_t b) {
+    return terms[a].flip_mask < terms[b].flip_mask;
+  });
+  for (int64_t k = 0; k < n_terms;) {
+    PauliGroup pg{};
+    pg.flip = terms[g.order[k]].flip_mask;
+    pg.first = (int32_t)k;
+    while (k < n_terms && pg.count < kTermsPerGroup && terms[g.order[k]].flip_mask == pg.flip) {
+      pg.sign[pg.count] = terms[g.order[k]].sign_mask;
+      pg.y[pg.count] = terms[g.order[k]].y_count;
+      pg.count++;
+      k++;
+    }
+    g.groups.push_back(pg);
+  }
+  return g;
+}
+
+void validate_terms(const qm_pauli *terms, int64_t n_terms, int n) {
+  if (n_terms < 0 || (n_terms > 0 && !terms)) fail(QM_ERR_ARG, "terms are null");
+  const uint64_t full = n >= 64 ? ~0ull : ((1ull << n) - 1);
+  for (int64_t i = 0; i < n_terms; i++)
+    if ((terms[i].flip_mask | terms[i].sign_mask) & ~full)
+      fail(QM_ERR_ARG, "Pauli string width must match the state.");
+}
+
+// A batch over `items` same-width states running one program.
+struct BatchJob {
+  int qubits = 0;
+  int precision = QM_C128;
+  std::vector<BatchStep> steps;
+  int64_t items = 0;
+  const double *pools = nullptr;
+  int64_t stride = 0;
+  std::vector<double> angles;  // [items][slots][2]
+  int32_t slots = 0;
+  const qm_pauli *terms = nullptr;
+  int64_t n_terms = 0;
+  bool want_mass = false;
+  const double *shot_u = nullptr;
+  int64_t shots = 0;
+  bool measure = false;
+};
+
+struct BatchOut {
+  std::vector<double> mass;   // [items]
+  std::vector<double> numer;  // [items][n_terms], original term order
+  int64_t fallbacks = 0;
+  double device_ms = 0;
+  int passes = 0;
+  double algorithmic_bytes = 0;
+  int64_t state_bytes = 0;
+};
+
+void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes, int64_t *consumed,
+               bool timed, BatchOut &out) {
+  const int n = job.qubits;
+  const FusedShape shape = batch_shape(job.precision, n);
+  if (shape.K == 0) fail(QM_ERR_UNSUPPORTED, "Batched execution needs at least 8 qubits.");
+  BatchProgram prog = plan_batch(job.steps, n, shape);
+  Groups g = group_terms(job.terms, job.n_terms);
+  const uint64_t len = 1ull << n;
+  const int64_t state_bytes = amp_bytes(job.precision) << n;
+
+  // Workgroup slots: as many states as can be resident at once, within the budget.
+  int64_t slots = std::min<int64_t>(job.items, (int64_t)c->compute_units * 2);
+  const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
+  const int64_t room = limit - c->resident_bytes - (64ll << 20);
+  slots = std::min<int64_t>(slots, room / state_bytes);
+  if (slots < 1) fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
+
+  std::vector<BatchItem> items((size_t)job.items);
+  for (int64_t i = 0; i < job.items; i++) {
+    BatchItem &it = items[(size_t)i];
+    it = BatchItem{};
+    it.len = len;
+    it.tiles = 1ull << (n - shape.K);
+    it.seg_begin = 0;
+    it.seg_count = (int32_t)prog.segs.size();
+    it.group_begin = 0;
+    it.group_count = (int32_t)g.groups.size();
+    it.numer_off = i * job.n_terms;
+    it.shots = job.shots;
+    it.shot_off = i * job.shots;
+    it.measure = job.measure ? 1 : 0;
+    it.want_mass = job.want_mass ? 1 : 0;
+  }
+
+  Arena ar;
+  const size_t o_states = ar.add((size_t)(slots * state_bytes));
+  const size_t o_items = ar.add(items.size() * sizeof(BatchItem));
+  const size_t o_passes = ar.add(prog.passes.size() * sizeof(DevPass));
+  const size_t o_ops = ar.add(std::max<size_t>(prog.ops.size(), 1) * sizeof(TileOp));
+  const size_t o_segs = ar.add(prog.segs.size() * sizeof(BatchSeg));
+  const size_t o_pools = ar.add(job.pools ? (size_t)(job.items * job.stride) * 8 : 0);
+  const size_t o_angles = ar.add(job.angles.size() * 8);
+  const size_t o_groups = ar.add(std::max<size_t>(g.groups.size(), 1) * sizeof(PauliGroup));
+  const size_t o_numer = ar.add((size_t)std::max<int64_t>(job.items * job.n_terms, 1) * 8);
+  const size_t o_mass = ar.add((size_t)job.items * 8);
+  const size_t o_shotu = ar.add((size_t)std::max<int64_t>(job.items * job.shots, 1) * 8);
+  const size_t o_samples = ar.add((size_t)std::max<int64_t>(job.items * job.shots, 1) * 8);
+  const size_t o_outcomes = ar.add((size_t)job.items * 8);
+  const size_t o_consumed = ar.add((size_t)job.items * 8);
+  const size_t o_err = ar.add((size_t)job.items * 4);
+  const size_t o_fb = ar.add(8);
+  if ((c->device_budget && c->resident_bytes + (int64_t)ar.size > c->device_budget) ||
+      c->resident_bytes + (int64_t)ar.size > c->device_memory)
+    fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
+  hipError_t e = hipMalloc(&ar.base, ar.size);
+  if (e != hipSuccess) {
+    ar.base = nullptr;
+    (void)hipGetLastError();
+    fail(QM_ERR_BUDGET, "HIP batch exceeds the free device memory.");
+  }
+  hipStream_t st = c->stream;
+  auto up = [&](size_t off, const void *src, size_t bytes) {
+    if (bytes) QM_HIP(hipMemcpyAsync(ar.at<char>(off), src, bytes, hipMemcpyHostToDevice, st));
+  };
+  up(o_items, items.data(), items.size() * sizeof(BatchItem));
+  up(o_passes, prog.passes.data(), prog.passes.size() * sizeof(DevPass));
+  up(o_ops, prog.ops.data(), prog.ops.size() * sizeof(TileOp));
+  up(o_segs, prog.segs.data(), prog.segs.size() * sizeof(BatchSeg));
+  if (job.pools) up(o_pools, job.pools, (size_t)(job.items * job.stride) * 8);
+  up(o_angles, job.angles.data(), job.angles.size() * 8);
+  up(o_groups, g.groups.data(), g.groups.size() * sizeof(PauliGroup));
+  if (job.shots) up(o_shotu, job.shot_u, (size_t)(job.items * job.shots) * 8);
+  QM_HIP(hipMemsetAsync(ar.at<char>(o_fb), 0, 8, st));
+
+  BatchArgs a{};
+  a.states = ar.at<void>(o_states);
+  a.slot_len = (int64_t)len;
+  a.keep = 0;
+  a.slots = job.slots;
+  a.items = ar.at<BatchItem>(o_items);
+  a.n_items = job.items;
+  a.passes = ar.at<DevPass>(o_passes);
+  a.ops = ar.at<TileOp>(o_ops);
+  a.segs = ar.at<BatchSeg>(o_segs);
+  a.pools = job.pools ? ar.at<double>(o_pools) : nullptr;
+  a.stride = job.stride;
+  a.angles = job.angles.empty() ? nullptr : ar.at<double>(o_angles);
+  a.groups = ar.at<PauliGroup>(o_groups);
+  a.numer = ar.at<double>(o_numer);
+  a.mass = ar.at<double>(o_mass);
+  a.shot_u = ar.at<double>(o_shotu);
+  a.samples = ar.at<int64_t>(o_samples);
+  a.outcomes = ar.at<int64_t>(o_outcomes);
+  a.consumed = ar.at<int64_t>(o_consumed);
+  a.err = ar.at<int32_t>(o_err);
+  a.fallbacks = ar.at<unsigned long long>(o_fb);
+
+  int64_t grid = slots;
+  if (timed) QM_HIP(hipEventRecord(c->ev_start, st));
+  launch_batch(c, job.precision, shape, a, &grid);
+  if (timed) QM_HIP(hipEventRecord(c->ev_stop, st));
+
+  std::vector<int32_t> err((size_t)job.items);
+  std::vector<double> numer_sorted((size_t)(job.items * job.n_terms));
+  out.mass.resize((size_t)job.items);
+  unsigned long long fb = 0;
+  auto down = [&](void *dst, size_t off, size_t bytes) {
+    if (bytes) QM_HIP(hipMemcpyAsync(dst, ar.at<char>(off), bytes, hipMemcpyDeviceToHost, st));
+  };
+  down(err.data(), o_err, err.size() * 4);
+  down(numer_sorted.data(), o_numer, numer_sorted.size() * 8);
+  down(out.mass.data(), o_mass, out.mass.size() * 8);
+  down(&fb, o_fb, 8);
+  if (samples && job.shots) down(samples, o_samples, (size_t)(job.items * job.shots) * 8);
+  if (outcomes && job.measure) down(outcomes, o_outcomes, (size_t)job.items * 8);
+  if (consumed) down(consumed, o_consumed, (size_t)job.items * 8);
+  QM_HIP(hipStreamSynchronize(st));
+  if (timed) {
+    float ms = 0;
+    QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    out.device_ms = ms;
+  }
+  for (int64_t i = 0; i < job.items; i++) {
+    const int code = err[(size_t)i];
+    if (code == QM_ERR_POOL) fail(code, "The uniform pool is exhausted.");
+    if (code == QM_ERR_MASS) fail(code, "The state has no finite positive probability mass.");
+    if (code) fail(code, "batched state failed");
+  }
+  out.numer.resize(numer_sorted.size());
+  for (int64_t i = 0; i < job.items; i++)
+    for (int64_t k = 0; k < job.n_terms; k++)
+      out.numer[(size_t)(i * job.n_terms + g.order[(size_t)k])] =
+          numer_sorted[(size_t)(i * job.n_terms + k)];
+  out.fallbacks = (int64_t)fb;
+  out.passes = (int)prog.passes.size();
+  out.state_bytes = state_bytes;
+  // Each pass reads and writes the state; the final stage reads it once per
+  // mass, per Pauli group and for the CDF.
+  const double finals = (job.want_mass || job.n_terms ? 1.0 : 0.0) + (double)g.groups.size() +
+                        (job.shots || job.measure ? 1.0 : 0.0);
+  out.algorithmic_bytes =
+      (double)job.items * (double)state_bytes * (2.0 * (double)prog.passes.size() + finals);
+}
+
+void fill_profile(qm_profile *p, const BatchOut &o) {
+  if (!p) return;
+  p->device_ms = o.device_ms;
+  p->state_bytes = o.state_bytes;
+  p->kernel_launches = 1;
+  p->passes = o.passes;
+  p->algorithmic_bytes = o.algorithmic_bytes;
+}
+
+bool valid_mass(double m) { return std::isfinite(m) && m > 0; }
+
+// ExpectationValue of one item's term (StateVector.Observations.cs:107-141).
+double expectation(const BatchOut &o, int64_t item, int64_t n_terms, int64_t t) {
+  const double m = o.mass[(size_t)item];
+  if (!valid_mass(m)) fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+  const double e = o.numer[(size_t)(item * n_terms + t)];
+  if (!std::isfinite(e)) fail(QM_ERR_NONFINITE, "The expectation value is nonfinite.");
+  return e / m;
+}
+
+// ExpectationEnergy (StateVector.Energy.cs:10-46): Neumaier sum in term order.
+double energy_of(const BatchOut &o, int64_t item, const double *coeffs, int64_t n_terms) {
+  if (!valid_mass(o.mass[(size_t)item]))
+    fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+  double sum = 0, corr = 0;
+  for (int64_t t = 0; t < n_terms; t++) {
+    if (coeffs[t] == 0) continue;
+    const double value = coeffs[t] * expectation(o, item, n_terms, t);
+    const double next = sum + value;
+    corr += std::fabs(sum) >= std::fabs(value) ? (sum - next) + value : (value - next) + sum;
+    sum = next;
+    if (!std::isfinite(sum) || !std::isfinite(corr))
+      fail(QM_ERR_NONFINITE, "The energy accumulation is nonfinite.");
+  }
+  const double e = sum + corr;
+  if (!std::isfinite(e)) fail(QM_ERR_NONFINITE, "The energy is nonfinite.");
+  return e;
+}
+
+// Template ops of a ParameterizedCircuit (ParameterizedCircuit.cs:68-96) as
+// batch steps: parameter rotations read their coefficients from the angle table.
+struct Template {
+  std::vector<BatchStep> steps;
+  std::vector<int32_t> rot_kind;   // per occurrence: 6 Rx, 7 Ry, 8 Rz
+  std::vector<int32_t> rot_param;  // per occurrence: parameter index
+};
+
+Template make_template(int n, const qm_op *ops, const int32_t *param_index, int64_t n_ops,
+                       int64_t n_params) {
+  if (n_ops < 0 || (n_ops > 0 && (!ops || !param_index))) fail(QM_ERR_ARG, "operations are null");
+  Template tp;
+  for (int64_t i = 0; i < n_ops; i++) {
+    const qm_op &op = ops[i];
+    const int32_t p = param_index[i];
+    BatchStep bs;
+    if (p < 0) {
+      validate_op(op, n);
+      if (!prepare_gate(op, &bs.gate)) continue;
+      tp.steps.push_back(bs);
+      continue;
+    }
+    if (p >= n_params) fail(QM_ERR_RANGE, "parameterIndex");
+    if (op.kind != 6 && op.kind != 7 && op.kind != 8) fail(QM_ERR_ARG, "Parameters bind Rx, Ry or Rz only.");
+    if (op.target < 0 || op.target >= n) fail(QM_ERR_RANGE, "target");
+    DevGate &g = bs.gate;
+    std::memset(&g, 0, sizeof(g));
+    g.target = op.target;
+    g.swap_other = -1;
+    g.kind = op.kind == 8 ? G_PROTZ : G_PROT;
+    g.aux0 = (int32_t)tp.rot_kind.size();
+    g.aux1 = op.kind == 6 ? G_RX : G_RY;
+    tp.rot_kind.push_back(op.kind);
+    tp.rot_param.push_back(p);
+    tp.steps.push_back(bs);
+  }
+  return tp;
+}
+
+// Coefficients of one rotation at angle a, as PreparedStateVectorGate.cs:26-45
+// forms them on the host: (cos, sin)(a/2).
+void put_angle(double *dst, double a) {
+  dst[0] = std::cos(a / 2);
+  dst[1] = std::sin(a / 2);
+}
+
+void check_parameters(const double *parameters, int64_t n_params) {
+  if (n_params < 0 || (n_params > 0 && !parameters)) fail(QM_ERR_ARG, "parameters are null");
+  for (int64_t i = 0; i < n_params; i++)
+    if (!std::isfinite(parameters[i])) fail(QM_ERR_RANGE, "Parameters must be finite.");
+}
+
+Ctx *batch_ctx(qm_ctx *h, qm_profile *profile) {
+  Ctx *c = reinterpret_cast<Ctx *>(h);
+  if (!c) fail(QM_ERR_ARG, "context is null");
+  use_device(c);
+  if (profile && !c->profiling)
+    fail(QM_ERR_STATE, "EnableProfiling must be selected when creating the HIP backend.");
+  return c;
+}
+
+void check_width(int32_t qubits, int32_t precision) {
+  if (qubits < 1 || qubits > 34) fail(QM_ERR_RANGE, "qubits must be between 1 and 34.");
+  if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+}
+
+}  // namespace
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+int qm_batch_min_qubits(int32_t precision, int32_t *qubits) {
+  return guarded([&] {
+    if (!qubits) fail(QM_ERR_ARG, "qubits is null");
+    if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+    int q = 1;
+    while (q <= 34 && batch_shape(precision, q).K == 0) q++;
+    *qubits = q;
+  });
+}
+
+int qm_trajectories_run(qm_ctx *h, int32_t qubits, int32_t precision, const qm_step *steps,
+                        int64_t n_steps, const qm_op *ops, int64_t n_ops, int64_t trajectories,
+                        const double *pools, int64_t pool_stride, int64_t shots,
+                        const double *shot_uniforms, int32_t measure_all, int64_t *samples,
+                        int64_t *outcomes, int64_t *consumed, int64_t *fallbacks,
+                        qm_profile *profile) {
+  return guarded([&] {
+    Ctx *c = batch_ctx(h, profile);
+    check_width(qubits, precision);
+    if (trajectories < 1) fail(QM_ERR_RANGE, "trajectories must be positive.");
+    if (n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "steps are null");
+    if (pool_stride < 0 || (pool_stride > 0 && !pools)) fail(QM_ERR_ARG, "pools are null");
+    if (shots < 0 || (shots > 0 && (!shot_uniforms || !samples))) fail(QM_ERR_ARG, "shot buffers are null");
+    if (measure_all && !outcomes) fail(QM_ERR_ARG, "outcomes are null");
+    std::lock_guard<std::recursive_mutex> lk(c->lock);
+    BatchJob job;
+    job.qubits = qubits;
+    job.precision = precision;
+    job.items = trajectories;
+    job.pools = pool_stride > 0 ? pools : nullptr;
+    job.stride = pool_stride;
+    job.shot_u = shot_uniforms;
+    job.shots = shots;
+    job.measure = measure_all != 0;
+    // NoisyCircuit.Execute (NoisyCircuit.cs:48-65): segments to gates, channels with
+    // p != 0 to branches (CpuNoiseChannelBackend.cs:10-83).
+    for (int64_t k = 0; k < n_steps; k++) {
+      const qm_step &s = steps[k];
+      if (!s.is_channel) {
+        if (s.op_begin < 0 || s.op_count < 0 || s.op_begin + s.op_count > n_ops)
+          fail(QM_ERR_ARG, "bad segment");
+        for (const DevGate &g : prepare_all(ops + s.op_begin, s.op_count, qubits)) {
+          BatchStep bs;
+          bs.gate = g;
+          job.steps.push_back(bs);
+        }
+        continue;
+      }
+      const double p = s.probability;
+      if (p == 0) continue;
+      if (s.target < 0 || s.target >= qubits) fail(QM_ERR_RANGE, "target");
+      if (!std::isfinite(p) || p < 0 || p > 1)
+        fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
+      if (s.noise_kind < 0 || s.noise_kind > 3) fail(QM_ERR_ARG, "unknown noise kind");
+      BatchStep bs;
+      if (s.noise_kind == 3) {
+        bs.damp_target = s.target;
+        bs.gamma = p;
+      } else {
+        DevGate &g = bs.gate;
+        std::memset(&g, 0, sizeof(g));
+        g.kind = G_NOISE;
+        g.target = s.target;
+        g.swap_other = -1;
+        g.aux0 = (s.noise_kind != 2 && p == 1) ? -1 : 0;  // flips at p == 1 draw nothing
+        g.aux1 = s.noise_kind;
+        g.c[0] = p;
+      }
+      job.steps.push_back(bs);
+    }
+    BatchOut out;
+    run_batch(c, job, samples, outcomes, consumed, profile != nullptr, out);
+    if (fallbacks) *fallbacks = out.fallbacks;
+    fill_profile(profile, out);
+  });
+}
+
+int qm_energy_gradient_batch(qm_ctx *h, int32_t qubits, int32_t precision, const qm_op *ops,
+                             const int32_t *param_index, int64_t n_ops, const double *parameters,
+                             int64_t n_params, const qm_pauli *terms, const double *coeffs,
+                             int64_t n_terms, double *energy, double *gradient,
+                             qm_profile *profile) {
+  return guarded([&] {
+    Ctx *c = batch_ctx(h, profile);
+    check_width(qubits, precision);
+    check_parameters(parameters, n_params);
+    validate_terms(terms, n_terms, qubits);
+    if (n_terms > 0 && !coeffs) fail(QM_ERR_ARG, "coefficients are null");
+    if (!energy || (n_params > 0 && !gradient)) fail(QM_ERR_ARG, "outputs are null");
+    std::lock_guard<std::recursive_mutex> lk(c->lock);
+    Template tp = make_template(qubits, ops, param_index, n_ops, n_params);
+    const int64_t R = (int64_t)tp.rot_kind.size();
+    // Item 0: the bound circuit; items 1 + 2r / 2 + 2r: rotation occurrence r
+    // shifted by +pi/2 / -pi/2 (VariationalEnergyEvaluator.cs:50-80).  The shift
+    // is folded into the rotation's angle: R(a) R(s) = R(a + s) for one axis.
+    BatchJob job;
+    job.qubits = qubits;
+    job.precision = precision;
+    job.steps = tp.steps;
+    job.items = 1 + 2 * R;
+    job.slots = (int32_t)std::max<int64_t>(R, 1);
+    job.angles.assign((size_t)(job.items * job.slots * 2), 0.0);
+    for (int64_t i = 0; i < job.items; i++)
+      for (int64_t r = 0; r < R; r++) {
+        double a = parameters[tp.rot_param[(size_t)r]];
+        if (i > 0 && (i - 1) / 2 == r) a += (i % 2 == 1) ? M_PI / 2 : -M_PI / 2;
+        put_angle(&job.angles[(size_t)((i * job.slots + r) * 2)], a);
+      }
+    job.terms = terms;
+    job.n_terms = n_terms;
+    job.want_mass = true;
+    BatchOut out;
+    run_batch(c, job, nullptr, nullptr, nullptr, profile != nullptr, out);
+    // Host assembly in the reference's order (VariationalEnergyEvaluator.cs:50-86).
+    *energy = energy_of(out, 0, coeffs, n_terms);
+    std::vector<double> grad((size_t)n_params, 0.0), corr((size_t)n_params, 0.0);
+    std::vector<double> plus((size_t)n_terms);
+    for (int64_t r = 0; r < R; r++) {
+      const int64_t param = tp.rot_param[(size_t)r];
+      const int64_t ip = 1 + 2 * r, im = 2 + 2 * r;
+      if (n_terms == 0 && !valid_mass(out.mass[(size_t)ip]))
+        fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+      for (int64_t t = 0; t < n_terms; t++) plus[(size_t)t] = expectation(out, ip, n_terms, t);
+      if (n_terms == 0 && !valid_mass(out.mass[(size_t)im]))
+        fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+      for (int64_t t = 0; t < n_terms; t++) {
+        const double minus = expectation(out, im, n_terms, t);
+        const double contribution = coeffs[t] * (0.5 * plus[(size_t)t] - 0.5 * minus);
+        const double prev = grad[(size_t)param];
+        const double next = prev + contribution;
+        corr[(size_t)param] += std::fabs(prev) >= std::fabs(contribution)
+                                   ? (prev - next) + contribution
+                                   : (contribution - next) + prev;
+        grad[(size_t)param] = next;
+        if (!std::isfinite(next) || !std::isfinite(corr[(size_t)param]))
+          fail(QM_ERR_NONFINITE, "The gradient accumulation is nonfinite.");
+      }
+    }
+    for (int64_t p = 0; p < n_params; p++) {
+      grad[(size_t)p] += corr[(size_t)p];
+      if (!std::isfinite(grad[(size_t)p])) fail(QM_ERR_NONFINITE, "The gradient is nonfinite.");
+      gradient[p] = grad[(size_t)p];
+    }
+    fill_profile(profile, out);
+  });
+}
+
+int qm_energy_batch(qm_ctx *h, int32_t qubits, int32_t precision, const qm_op *ops,
+                    const int32_t *param_index, int64_t n_ops, const double *parameter_sets,
+                    int64_t n_params, int64_t n_sets, const qm_pauli *terms, const double *coeffs,
+                    int64_t n_terms, double *energies, qm_profile *profile) {
+  return guarded([&] {
+    Ctx *c = batch_ctx(h, profile);
+    check_width(qubits, precision);
+    if (n_sets < 1) fail(QM_ERR_RANGE, "n_sets must be positive.");
+    check_parameters(parameter_sets, n_params * n_sets);
+    validate_terms(terms, n_terms, qubits);
+    if (n_terms > 0 && !coeffs) fail(QM_ERR_ARG, "coefficients are null");
+    if (!energies) fail(QM_ERR_ARG, "energies is null");
+    std::lock_guard<std::recursive_mutex> lk(c->lock);
+    Template tp = make_template(qubits, ops, param_index, n_ops, n_params);
+    const int64_t R = (int64_t)tp.rot_kind.size();
+    BatchJob job;
+    job.qubits = qubits;
+    job.precision = precision;
+    job.steps = tp.steps;
+    job.items = n_sets;
+    job.slots = (int32_t)std::max<int64_t>(R, 1);
+    job.angles.assign((size_t)(job.items * job.slots * 2), 0.0);
+    for (int64_t i = 0; i < n_sets; i++)
+      for (int64_t r = 0; r < R; r++)
+        put_angle(&job.angles[(size_t)((i * job.slots + r) * 2)],
+                  parameter_sets[i * n_params + tp.rot_param[(size_t)r]]);
+    job.terms = terms;
+    job.n_terms = n_terms;
+    job.want_mass = true;
+    BatchOut out;
+    run_batch(c, job, nullptr, nullptr, nullptr, profile != nullptr, out);
+    for (int64_t i = 0; i < n_sets; i++) energies[i] = energy_of(out, i, coeffs, n_terms);
+    fill_profile(profile, out);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Batched program execution (batch.hip): kernel arguments and launcher.
+#pragma once
+
+#include <cstdint>
+
+#include "dev_reduce.hpp"
+#include "plan.hpp"
+
+namespace qm {
+
+struct Ctx;
+
+// One state of a batch: where it lives, which segments of the program it runs
+// and what is measured on it afterwards.
+struct BatchItem {
+  int64_t state_off;    // amplitude offset of the state (kept states only)
+  uint64_t len;         // amplitudes, 2^n with n >= K
+  uint64_t tiles;       // 2^(n - K)
+  int32_t seg_begin;    // program segments run by this item
+  int32_t seg_count;
+  int32_t group_begin;  // Pauli groups evaluated after the program
+  int32_t group_count;
+  int64_t numer_off;    // first numerator slot of the item
+  int64_t shots;        // terminal shots
+  int64_t shot_off;     // first uniform / sample slot of the item
+  int32_t measure;      // 1: a MeasureAll draw from the pool after the shots
+  int32_t want_mass;    // 1: write the total mass
+};
+
+struct BatchArgs {
+  void *states;          // state buffers: per item (keep) or per workgroup slot
+  int64_t slot_len;      // amplitudes per workgroup slot (keep == 0)
+  int32_t keep;          // 1: item states live at states + item.state_off
+  int32_t slots;         // angle-table entries per item
+  const BatchItem *items;
+  int64_t n_items;
+  const DevPass *passes;
+  const TileOp *ops;
+  const BatchSeg *segs;
+  const double *pools;   // [items][stride] uniform pools, or null
+  int64_t stride;
+  const double *angles;  // [items][slots][2] rotation coefficients, or null
+  const PauliGroup *groups;
+  double *numer;         // Pauli numerators at item.numer_off + group.first + k
+  double *mass;          // [items]
+  const double *shot_u;  // uniforms at item.shot_off
+  int64_t *samples;      // samples at item.shot_off
+  int64_t *outcomes;     // [items]
+  int64_t *consumed;     // [items] draws consumed from the pool
+  int32_t *err;          // [items] status per item
+  unsigned long long *fallbacks;  // margin fallbacks (sequential re-decisions)
+};
+
+// Largest compiled batch tile shape with K <= qubits (K = 0: none).
+FusedShape batch_shape(int precision, int qubits);
+
+// Launches the batch kernel; *grid in: cap on workgroups for slot buffers
+// (0: none), out: workgroups launched.
+void launch_batch(Ctx *c, int precision, const FusedShape &shape, const BatchArgs &args,
+                  int64_t *grid);
+
+}  // namespace qm

@@ -14,6 +14,7 @@
 #include "dev_types.hpp"
 #include "gate_math.hpp"
 #include "qm_internal.hpp"
+#include "dev_reduce.hpp"
 
 namespace qm {
 
@@ -259,45 +260,6 @@ void launch_convert(const State *src, State *dst, hipStream_t st) {
 // FP64 reductions.  Block partials in a fixed order, then one block folds the
 // partials in a fixed order: results are deterministic run to run.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ double prob_of(const T &a) {
-  const double re = (double)a.re, im = (double)a.im;
-  return re * re + im * im;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ __forceinline__ double block_sum(double v, double *sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = 0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
-  return r;  // valid in thread 0
-}
-
-__device__ __forceinline__ long long block_max(long long v, long long *sh) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    long long u = __shfl_xor(v, o, 64);
-    v = u > v ? u : v;
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  long long r = -1;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = sh[i] > r ? sh[i] : r;
-  return r;
-}
 
 // mode 0: total mass + last supported index; mode 1: masked mass; mode 2:
 // masses of bit 0 / bit 1 of `mask` (stored as partial pairs).
@@ -450,14 +412,6 @@ void serial_bit_masses(State *s, int target, double *m0, double *m1, hipStream_t
 
 // Pauli numerators (StateVector.Observations.cs:269-286): for each term of a
 // flip-mask group, sum_j (-1)^popcount(j & sign) f_y(conj(a[j ^ flip]) * a[j]).
-constexpr int kTermsPerGroup = 8;
-struct PauliGroup {
-  uint64_t flip;
-  int32_t count;
-  int32_t first;  // index of the group's first term in the output order
-  uint64_t sign[kTermsPerGroup];
-  int32_t y[kTermsPerGroup];
-};
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pauli(const T *__restrict__ s, uint64_t n, uint64_t per,

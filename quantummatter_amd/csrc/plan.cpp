@@ -172,7 +172,7 @@ std::vector<PGate> expand(const std::vector<DevGate> &in) {
       out.push_back({x, true, ((uint64_t)1 << a) | ((uint64_t)1 << b)});
       continue;
     }
-    bool pair = !(g.kind == G_DIAG || g.kind == G_DIAG1);
+    bool pair = !(g.kind == G_DIAG || g.kind == G_DIAG1 || g.kind == G_PROTZ);
     out.push_back({g, pair, ((uint64_t)1 << g.target) | g.ctrl_mask});
   }
   return out;
@@ -467,6 +467,8 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         TileOp t;
         std::memset(&t, 0, sizeof(t));
         t.kind = x.g.kind;
+        t.aux0 = x.g.aux0;
+        t.aux1 = x.g.aux1;
         t.rbit = -1;
         for (int i = 0; i < 8; i++) t.c[i] = x.g.c[i];
         for (int q = 0; q < n; q++) {
@@ -489,7 +491,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         } else {
           t.dlane = 1u << rank_in(~rset & ((1u << K) - 1), lp);
         }
-        if (t.rbit >= 0 && t.reg_cmask != 0 && t.kind != G_X) {
+        if (t.rbit >= 0 && t.reg_cmask != 0 && t.kind != G_X && t.kind < G_PHASEPROD) {
           // Register-controlled: the kernel runs the general 2x2 form, selecting the
           // identity for pairs whose control bits are clear (exact either way).
           to_matrix(x.g, t.c);
@@ -515,10 +517,90 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     for (int ph = 0; ph < d.nphases; ph++)
       if (d.phase_diag_count[ph]) d.phase_diag_begin[ph] += tail_base;
     pass.ops.insert(pass.ops.end(), tail.begin(), tail.end());
+    pass.rsets = rsets;
     plan.passes.push_back(std::move(pass));
     pending.swap(deferred);
   }
   return plan;
+}
+
+namespace {
+
+// Marks pass p to accumulate the masses of `qubit`'s zero and one halves after
+// its last phase, in that phase's register/lane/outside coordinates.
+void set_mass_epilogue(FusedPass &p, int qubit) {
+  DevPass &d = p.dev;
+  const int K = popc(d.qmask);
+  int local = -1;
+  for (int k = 0; k < K; k++)
+    if (d.qpos[k] == qubit) local = k;
+  d.mass_on = 1;
+  d.mass_dreg = d.mass_dlane = 0;
+  d.mass_dout = 0;
+  if (local < 0) {
+    d.mass_dout = (uint64_t)1 << qubit;
+    return;
+  }
+  const uint32_t rset = p.rsets.back();
+  if (rset & (1u << local))
+    d.mass_dreg = 1u << rank_in(rset, local);
+  else
+    d.mass_dlane = 1u << rank_in(~rset & ((1u << K) - 1), local);
+}
+
+}  // namespace
+
+BatchProgram plan_batch(const std::vector<BatchStep> &steps, int n, const FusedShape &shape) {
+  BatchProgram prog;
+  prog.shape = shape;
+  prog.n = n;
+  std::vector<DevGate> seg;
+  int32_t draws = 0;
+  auto close = [&](int32_t damp_target, double gamma) {
+    if (seg.empty()) {
+      // A unit phase keeps the pass (the state is still read, reset and written).
+      DevGate g{};
+      g.kind = G_DIAG1;
+      g.swap_other = -1;
+      g.c[0] = 1, g.c[6] = 1;
+      seg.push_back(g);
+    }
+    FusedPlan plan = plan_fused(seg, n, shape);
+    if (damp_target >= 0) set_mass_epilogue(plan.passes.back(), damp_target);
+    BatchSeg bs{};
+    bs.pass_begin = (int32_t)prog.passes.size();
+    bs.pass_count = (int32_t)plan.passes.size();
+    bs.draws = draws;
+    bs.damp_target = damp_target;
+    bs.gamma = gamma;
+    for (FusedPass &p : plan.passes) {
+      p.dev.op_offset = (int32_t)prog.ops.size();
+      prog.passes.push_back(p.dev);
+      prog.ops.insert(prog.ops.end(), p.ops.begin(), p.ops.end());
+    }
+    prog.source_gates += plan.source_gates;
+    prog.segs.push_back(bs);
+    prog.draws_static += draws;
+    seg.clear();
+    draws = 0;
+  };
+  for (const BatchStep &st : steps) {
+    if (st.damp_target >= 0) {
+      close(st.damp_target, st.gamma);
+      prog.damps++;
+      DevGate g{};
+      g.kind = G_DAMP;
+      g.target = st.damp_target;
+      g.swap_other = -1;
+      seg.push_back(g);
+      continue;
+    }
+    DevGate g = st.gate;
+    if (g.kind == G_NOISE && g.aux0 >= 0) g.aux0 = draws++;
+    seg.push_back(g);
+  }
+  close(-1, 0);
+  return prog;
 }
 
 // ---------------------------------------------------------------------------

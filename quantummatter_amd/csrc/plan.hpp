@@ -28,6 +28,13 @@ enum GKind : int32_t {
   // and controls are lane bits or outside the tile; their per-lane product is
   // formed once and applied to every register in one multiplication.
   G_PHASEPROD = 9,
+  // Batched passes only: ops resolved per state at run time.
+  G_NOISE = 10,  // Pauli channel branch drawn from the state's uniform pool (aux0 = draw offset
+                 // within the segment or -1 for no draw, aux1 = noise kind, c[0] = probability)
+  G_DAMP = 11,   // amplitude-damping Kraus branch decided on the device before the pass
+  G_PROT = 12,   // rotation whose (c, s) come from the state's angle table (aux0 = table slot,
+                 // aux1 = G_RX or G_RY)
+  G_PROTZ = 13,  // Rz whose D = (cos, sin)(theta/2) comes from the angle table (A = conj(D))
 };
 
 struct DevGate {
@@ -37,6 +44,8 @@ struct DevGate {
   int32_t pad;
   uint64_t ctrl_mask;
   double c[8];  // coefficients, layout per kind
+  int32_t aux0;  // dynamic kinds: see GKind
+  int32_t aux1;
 };
 
 // Validates one GateOperation as QuantumCircuit.Add does (QuantumCircuit.cs:97-126);
@@ -78,8 +87,10 @@ struct TileOp {
   uint32_t dlane;       // diagonal target as a thread-index mask (0 if not a lane bit)
   uint64_t dout;        // diagonal target as a global mask outside the tile (0 if inside)
   double c[8];
+  int32_t aux0;         // dynamic kinds: see GKind
+  int32_t aux1;
 };
-static_assert(sizeof(TileOp) == 104, "TileOp layout");
+static_assert(sizeof(TileOp) == 112, "TileOp layout");
 
 struct DevPass {
   uint64_t qmask;                     // global mask of the tile qubits
@@ -97,6 +108,13 @@ struct DevPass {
   // per-lane phase product (indices relative to op_offset; count 0 = none).
   int32_t phase_diag_begin[kMaxPhases];
   int32_t phase_diag_count[kMaxPhases];
+  // Batched passes: masses of one qubit's zero and one halves accumulated after
+  // the last phase (mass_on), its position relative to the last phase's mapping.
+  int32_t mass_on;
+  uint32_t mass_dreg;
+  uint32_t mass_dlane;
+  uint32_t mass_pad;
+  uint64_t mass_dout;
   uint16_t regloc[kMaxPhases + 1][kMaxRegs];  // local index of register r per mapping
   uint16_t regswz[kMaxPhases + 1][kMaxRegs];  // lds_swizzle(regloc), see lds_swizzle()
   uint32_t lane_swz[kMaxPhases + 1][16];      // lds_swizzle(1 << lanepos) of thread bit k
@@ -158,6 +176,7 @@ struct FusedPass {
   std::vector<TileOp> ops;  // phases contiguous
   int gates = 0;            // source gates applied (after swap expansion)
   int transposes = 0;       // LDS round trips
+  std::vector<uint32_t> rsets;  // register set (tile-local mask) of each phase
 };
 
 struct FusedPlan {
@@ -171,6 +190,43 @@ struct FusedPlan {
 
 // Plans prepared gates (identity removed) for an n-qubit state, n >= shape.K.
 FusedPlan plan_fused(const std::vector<DevGate> &gates, int n, const FusedShape &shape);
+
+// ---------------------------------------------------------------------------
+// Batched programs: one program run on many same-width states, each state by
+// one workgroup.  A program is a list of segments; each segment is a fused
+// plan whose passes may hold ops resolved per state (noise branches drawn from
+// the state's own uniform pool, rotations from its angle table).  A segment
+// that ends in amplitude damping accumulates the masses of the damping target
+// in its last pass; the branch is decided on the device and applied by a
+// G_DAMP op at the start of the next segment.
+// ---------------------------------------------------------------------------
+struct BatchSeg {
+  int32_t pass_begin;
+  int32_t pass_count;
+  int32_t draws;        // static draws of the segment's noise ops
+  int32_t damp_target;  // -1: no damping after the segment
+  double gamma;
+};
+
+struct BatchStep {
+  // One gate (static or dynamic), or a damping channel when damp_target >= 0.
+  DevGate gate;
+  int32_t damp_target = -1;
+  double gamma = 0;
+};
+
+struct BatchProgram {
+  FusedShape shape{};
+  int n = 0;
+  std::vector<DevPass> passes;
+  std::vector<TileOp> ops;
+  std::vector<BatchSeg> segs;
+  int64_t draws_static = 0;   // draws of all noise ops (damping draws are dynamic)
+  int64_t damps = 0;
+  int64_t source_gates = 0;
+};
+
+BatchProgram plan_batch(const std::vector<BatchStep> &steps, int n, const FusedShape &shape);
 
 // Host reference of the kernels, used by the planner's CPU tests: applies a
 // plan / one gate to an interleaved complex128 state with the kernels'

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -28,6 +30,29 @@ struct Status {
 void hip_check(hipError_t e, const char *what);
 
 #define QM_HIP(call) ::qm::hip_check((call), #call)
+
+// Runs fn, turning a thrown Status (or any exception) into a status code and
+// the thread's last-error text: nothing is thrown across the C boundary.
+template <typename Fn>
+int guarded(Fn &&fn) {
+  try {
+    clear_error();
+    fn();
+    return QM_OK;
+  } catch (const Status &s) {
+    set_error(s.msg);
+    return s.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return QM_ERR_BUDGET;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return QM_ERR_DEVICE;
+  } catch (...) {
+    set_error("unknown failure");
+    return QM_ERR_DEVICE;
+  }
+}
 
 struct Ctx {
   int device = 0;
@@ -66,6 +91,11 @@ struct State {
 };
 
 inline int64_t amp_bytes(int precision) { return precision == QM_C128 ? 16 : 8; }
+
+// Shared host helpers (api.hip).
+void use_device(Ctx *c);  // checks the context and makes its device current
+void check_cancel(const volatile int32_t *cancel);
+std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n);  // validated, identity dropped
 
 // Scratch management on the context's stream.
 double *ctx_partials(Ctx *c, int64_t n);
