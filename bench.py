@@ -10,8 +10,10 @@ synchronised), with the state already in HBM.
 Also reported: the dominant kernel's roofline (the fused tile pass: 2 x 16 B x
 2^30 algorithmic bytes per launch, average duration from HIP events recorded
 by the library on its stream around the timed sequence), and the CPU baseline
-(the oracle's scalar restatement of the reference path, timed on a bounded
-sample on this host and scaled to the same unit).
+(the reference's parallel AVX-512 CPU path, restated in C++ in oracle/, timed
+on a bounded sample on this host's cores and scaled to the same unit).  The
+batched forms (noisy trajectories, parameter-shift VQE gradients) are reported
+beside it under "batched".
 """
 import argparse
 import json
@@ -50,33 +52,113 @@ def pmc_traffic(path):
     return None
 
 
-def cpu_baseline(n_target, layers_target):
-    """Scalar oracle on a bounded sample: 22 qubits, 2 layers, scaled by 2^(30-22)."""
+def _cpu_flags():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_target):
+    """The reference's parallel AVX-512 CPU path (restated in C++: workers over
+    contiguous pair ranges, AVX-512 kernels, 32-gate/512-pair blocking) on a
+    bounded sample: 26 qubits x 2 layers of the same generator, scaled by
+    2^(n_target - 26) to the target width's gate-layers/s."""
+    import ctypes as C
+    import numpy as np
     import oracle
     from quantummatter_amd import workloads
-    n_s, l_s = 22, 2
-    c = workloads.random_layer_circuit(n_s, l_s, seed=1)
-    ops = c.to_ops()
-    oracle.lib()
+    n_s, l_s = 26, 2
+    workers = max(1, min(16, os.cpu_count() or 1))
+    ops = np.ascontiguousarray(workloads.random_layer_circuit(n_s, l_s, seed=1).to_ops())
+    state = oracle.zero_state(n_s)
+    lib = oracle.lib()
+    P = C.c_void_p
     t0 = time.perf_counter()
-    oracle.execute(n_s, ops)
+    rc = lib.qmo_parallel_execute(n_s, ops.ctypes.data_as(P), len(ops), state.ctypes.data_as(P),
+                                  workers, 1, 1, 1)
     dt = time.perf_counter() - t0
+    if rc != 0:
+        return None
     per_layer_at_target = dt / l_s * (2 ** (n_target - n_s))
+    avx512 = "avx512f" in _cpu_flags()
     return {
         "value": 1.0 / per_layer_at_target,
         "unit": "gate-layers/s",
-        "cores": 1,
+        "cores": workers,
         "kind": "port",
-        "sample": f"scalar oracle, {n_s}q x {l_s} layers of the same generator ({len(ops)} gates), "
-                  f"{dt:.2f} s, scaled linearly in amplitudes to {n_target}q",
+        "sample": f"parallel CPU path ({'AVX-512' if avx512 else 'scalar fallback, no AVX-512 on host'}"
+                  f", blocking on, {workers} workers, {_cpu_model()}), {n_s}q x {l_s} layers "
+                  f"({len(ops)} gates) in {dt:.2f} s, scaled linearly in amplitudes to {n_target}q",
     }
+
+
+def batched_extras(be):
+    """Batched forms: 4096 noisy 16q trajectories x 30 layers with 1024 shots each,
+    and parameter-shift VQE energy+gradient (HEA, 6 layers, 256 Pauli terms)."""
+    import numpy as np
+    from quantummatter_amd import workloads
+    from quantummatter_amd.backend import C64, C128
+    from quantummatter_amd.batched import HipBatchedEnergyEvaluator, HipTrajectoryRunner
+    out = {}
+    n, L, T, shots = 16, 30, 4096, 1024
+    nc = workloads.noisy_layer_circuit(n, L, seed=3)
+    steps, _ = nc.packed()
+    stride = int(np.sum(steps["is_channel"] == 1)) + 1
+    pools = workloads.uniform_pools(7, T, stride)
+    su = workloads.uniforms(8, T * shots).reshape(T, shots)
+    for name, prec in (("c128", C128), ("c64", C64)):
+        r = HipTrajectoryRunner(be, prec)
+        r.Run(nc, pools[:64], shots, su[:64], True)
+        t0 = time.perf_counter()
+        res = r.Run(nc, pools, shots, su, True, profile=True)
+        wall = time.perf_counter() - t0
+        p = res.profile
+        gbps = p.AlgorithmicBytes / (p.DeviceEvolutionMs * 1e-3) / 1e9
+        out[f"trajectories_16q_30l_{name}"] = {
+            "trajectories_per_s": T / wall, "wall_s": wall, "device_ms": p.DeviceEvolutionMs,
+            "passes_per_trajectory": p.Passes, "algorithmic_GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBS,
+            "shots_per_trajectory": shots, "boundary_fallbacks": res.boundary_fallbacks}
+    for q in (12, 16, 20):
+        circ = workloads.hardware_efficient_ansatz(q, 6)
+        theta = workloads.random_parameters(circ.ParameterCount, 1)
+        ham = workloads.random_hamiltonian(q, 256, 2)
+        for name, prec in (("c128", C128), ("c64", C64)):
+            ev = HipBatchedEnergyEvaluator(be, prec)
+            ev.EvaluateWithGradient(circ, ham, theta)
+            t0 = time.perf_counter()
+            ev.EvaluateWithGradient(circ, ham, theta, profile=True)
+            wall = time.perf_counter() - t0
+            p = ev.last_profile
+            circuits = 1 + 2 * circ.ParameterCount
+            gbps = p.AlgorithmicBytes / (p.DeviceEvolutionMs * 1e-3) / 1e9
+            out[f"vqe_gradient_{q}q_{name}"] = {
+                "circuits_per_s": circuits / wall, "circuits": circuits, "wall_s": wall,
+                "device_ms": p.DeviceEvolutionMs, "algorithmic_GBps": gbps,
+                "hbm_frac": gbps / HBM_PEAK_GBS}
+    return out
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note "
-                                   "(DESIGN.md), --gpus must be 1"}))
+        print(json.dumps({"error": "this build measures one GPU (--gpus 1); sharding a state or "
+                                   "a batch across GPUs is not built"}))
         sys.exit(2)
     from quantummatter_amd import workloads
     from quantummatter_amd.backend import (C64, C128, SCHED_FUSED, SCHED_PER_GATE, HipBackendOptions,
@@ -116,6 +198,8 @@ def main():
                     "launches_per_layer": p2.KernelLaunches / 2,
                     "algorithmic_GBps": p2.AlgorithmicBytes / (p2.DeviceEvolutionMs * 1e-3) / 1e9,
                 }
+        if not args.no_extras:
+            extras["batched"] = batched_extras(be)
     traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"fused_pass_{args.precision}_{n}q_pmc.txt"))
     line = {
         "metric": "30q c128 gate-layers/s + % HBM peak; served circuits/s with p50/p99 latency",
@@ -142,7 +226,7 @@ def main():
     }
     line.update(extras)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(n, L)
+        line["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(line), flush=True)
 
 
