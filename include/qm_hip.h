@@ -266,6 +266,73 @@ int qm_energy_batch(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_op 
                     int64_t n_params, int64_t n_sets, const qm_pauli *terms, const double *coeffs,
                     int64_t n_terms, double *energies, qm_profile *profile);
 
+/* ---------------------------------------------------------------------------
+ * Request server (no counterpart in the reference, which runs one circuit per
+ * call): requests of 1..34 qubits are admitted in arrival order when their
+ * state fits the HBM budget beside everything running; widths up to
+ * batch_max_qubits are batched (one workgroup per state, one launch per tile
+ * shape, continuously re-formed from whatever waits), wider ones run on
+ * concurrent full-chip lanes.  Each lane has its own stream, so work overlaps.
+ * Results use the same kernels and rules as qm_state_sample /
+ * qm_state_expectations / a downloaded resident state.
+ * ------------------------------------------------------------------------- */
+#define QM_REQ_AMPLITUDES 0  /* out: 2^qubits amplitudes in the request's precision */
+#define QM_REQ_SHOTS 1       /* out: int64_t[shots], StateVectorSimulator.Run's CDF rule */
+#define QM_REQ_EXPECTATION 2 /* out: double[n_terms], StateVector.ExpectationValue per term */
+
+typedef struct qm_server_options {
+  int32_t device_index;
+  int32_t full_lanes;          /* concurrent full-chip lanes (0: 2) */
+  int32_t batch_max_qubits;    /* widths batched (0: 16) */
+  int32_t max_batch;           /* states per batched launch (0: 1024) */
+  int64_t device_budget_bytes; /* admission budget (0: device memory less 4 GiB) */
+} qm_server_options;
+
+typedef struct qm_request {
+  int32_t qubits;
+  int32_t precision;
+  int32_t kind;
+  int32_t pad;
+  const qm_op *ops;
+  int64_t n_ops;
+  int64_t shots;
+  const double *uniforms; /* [shots], copied at submit */
+  const qm_pauli *terms;  /* [n_terms], copied at submit */
+  int64_t n_terms;
+  void *out; /* written before the request is reported done; valid until then */
+} qm_request;
+
+typedef struct qm_request_status {
+  int64_t id;
+  int32_t status;
+  int32_t lane;      /* 0: batch lane, 1..: full-chip lanes */
+  double submit_ms;  /* server clock */
+  double start_ms;
+  double done_ms;
+  int64_t fallbacks; /* margin re-decisions of the launch that served it */
+} qm_request_status;
+
+typedef struct qm_server_stats {
+  int64_t submitted, completed, failed, pending;
+  int64_t reserved_bytes, peak_reserved_bytes, budget_bytes;
+  int64_t batches, batched_requests;
+  double p50_ms, p99_ms; /* latency, done - submit, over completed requests */
+  double span_ms;        /* first submit to last completion */
+} qm_server_stats;
+
+typedef struct qm_server qm_server;
+
+int qm_server_create(const qm_server_options *options, qm_server **server);
+/* Validates and copies the request (ops prepared now); *id identifies it. */
+int qm_server_submit(qm_server *server, const qm_request *request, int64_t *id);
+/* Blocks until the request is done; returns its status (message in qm_last_error). */
+int qm_server_wait(qm_server *server, int64_t id, qm_request_status *status);
+int qm_server_drain(qm_server *server);
+int qm_server_get_stats(qm_server *server, qm_server_stats *stats);
+int qm_server_reset_stats(qm_server *server);
+/* Waits for every submitted request, stops the lanes and frees the contexts. */
+int qm_server_destroy(qm_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,13 @@ SIGNATURES = {
     "qm_trajectories_run": [P, I32, I32, P, I64, P, I64, I64, P, I64, I64, P, I32, P, P, P, P, P],
     "qm_energy_gradient_batch": [P, I32, I32, P, P, I64, P, I64, P, P, I64, P, P, P],
     "qm_energy_batch": [P, I32, I32, P, P, I64, P, I64, I64, P, P, I64, P, P],
+    "qm_server_create": [P, P],
+    "qm_server_submit": [P, P, P],
+    "qm_server_wait": [P, I64, P],
+    "qm_server_drain": [P],
+    "qm_server_get_stats": [P, P],
+    "qm_server_reset_stats": [P],
+    "qm_server_destroy": [P],
 }
 _RESTYPE = {"qm_last_error": C.c_char_p, "qm_abi_version": C.c_int32}
 

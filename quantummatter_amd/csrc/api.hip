@@ -108,14 +108,9 @@ std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
   return gates;
 }
 
-struct ExecStats {
-  int launches = 0;
-  int passes = 0;
-  double bytes = 0;
-};
 
 // Runs prepared gates on a resident state with the chosen schedule.
-static ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule,
+ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule,
                            const volatile int32_t *cancel) {
   Ctx *c = s->ctx;
   ExecStats st;
@@ -276,7 +271,11 @@ int qm_ctx_resident_bytes(qm_ctx *h, int64_t *bytes) {
   });
 }
 
-static State *state_new(Ctx *c, int qubits, int precision) {
+}  // extern "C"
+
+namespace qm {
+
+State *state_new(Ctx *c, int qubits, int precision) {
   if (qubits < 1 || qubits > 34) fail(QM_ERR_RANGE, "qubits must be between 1 and 34.");
   if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
   const int64_t bytes = amp_bytes(precision) << qubits;
@@ -299,13 +298,17 @@ static State *state_new(Ctx *c, int qubits, int precision) {
   return s;
 }
 
-static void state_free(State *s) {
+void state_free(State *s) {
   Ctx *c = s->ctx;
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(s->data);
   c->resident_bytes -= s->bytes;
   delete s;
 }
+
+}  // namespace qm
+
+extern "C" {
 
 int qm_execute(qm_ctx *h, int32_t qubits, const qm_op *ops, int64_t n_ops, double *amps,
                int64_t n_amps, const volatile int32_t *cancel, qm_profile *profile) {

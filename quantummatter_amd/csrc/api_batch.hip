@@ -13,6 +13,29 @@
 #include "qm_internal.hpp"
 
 namespace qm {
+
+TermGroups group_terms(const qm_pauli *terms, int64_t n_terms) {
+  TermGroups g;
+  g.order.resize((size_t)n_terms);
+  std::iota(g.order.begin(), g.order.end(), 0);
+  std::stable_sort(g.order.begin(), g.order.end(), [&](int64_t a, int64_t b) {
+    return terms[a].flip_mask < terms[b].flip_mask;
+  });
+  for (int64_t k = 0; k < n_terms;) {
+    PauliGroup pg{};
+    pg.flip = terms[g.order[k]].flip_mask;
+    pg.first = (int32_t)k;
+    while (k < n_terms && pg.count < kTermsPerGroup && terms[g.order[k]].flip_mask == pg.flip) {
+      pg.sign[pg.count] = terms[g.order[k]].sign_mask;
+      pg.y[pg.count] = terms[g.order[k]].y_count;
+      pg.count++;
+      k++;
+    }
+    g.groups.push_back(pg);
+  }
+  return g;
+}
+
 namespace {
 
 // One device allocation for everything a batched call uploads and reads back.
@@ -33,33 +56,117 @@ struct Arena {
   }
 };
 
-struct Groups {
-  std::vector<PauliGroup> groups;
-  std::vector<int64_t> order;  // sorted position k -> original term index
-};
+}  // namespace
 
-// Terms grouped by flip mask (stable in term order), at most kTermsPerGroup each.
-Groups group_terms(const qm_pauli *terms, int64_t n_terms) {
-  Groups g;
-  g.order.resize((size_t)n_terms);
-  std::iota(g.order.begin(), g.order.end(), 0);
-  std::stable_sort(g.order.begin(), g.order.end(), [&](int64_t a, int64_t b) {
-    return terms[a].flip_mask < terms[b].flip_mask;
-  });
-  for (int64_t k = 0; k < n_terms;) {
-    PauliGroup pg{};
-    pg.flip = terms[g.order[k]].flip_mask;
-    pg.first = (int32_t)k;
-    while (k < n_terms && pg.count < kTermsPerGroup && terms[g.order[k]].flip_mask == pg.flip) {
-      pg.sign[pg.count] = terms[g.order[k]].sign_mask;
-      pg.y[pg.count] = terms[g.order[k]].y_count;
-      pg.count++;
-      k++;
-    }
-    g.groups.push_back(pg);
+void run_batch_launch(Ctx *c, BatchLaunch &L) {
+  const int64_t n_items = (int64_t)L.items.size();
+  const int64_t ab = amp_bytes(L.precision);
+  int64_t slots = 0;
+  if (!L.keep) {
+    // Workgroup slots: as many states as can be resident at once, within the budget.
+    slots = std::min<int64_t>(n_items, (int64_t)c->compute_units * 2);
+    const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
+    const int64_t room = limit - c->resident_bytes - (64ll << 20);
+    slots = std::min<int64_t>(slots, room / std::max<int64_t>(L.slot_len * ab, 1));
+    if (slots < 1) fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
   }
-  return g;
+  const int64_t state_amps = L.keep ? L.keep_len : slots * L.slot_len;
+  Arena ar;
+  const size_t o_states = ar.add((size_t)(state_amps * ab));
+  const size_t o_items = ar.add(L.items.size() * sizeof(BatchItem));
+  const size_t o_passes = ar.add(L.prog.passes.size() * sizeof(DevPass));
+  const size_t o_ops = ar.add(std::max<size_t>(L.prog.ops.size(), 1) * sizeof(TileOp));
+  const size_t o_segs = ar.add(L.prog.segs.size() * sizeof(BatchSeg));
+  const size_t o_pools = ar.add(L.pools ? (size_t)(n_items * L.stride) * 8 : 0);
+  const size_t o_angles = ar.add(L.angles.size() * 8);
+  const size_t o_groups = ar.add(std::max<size_t>(L.groups.size(), 1) * sizeof(PauliGroup));
+  const size_t o_numer = ar.add((size_t)std::max<int64_t>(L.n_numer, 1) * 8);
+  const size_t o_mass = ar.add((size_t)n_items * 8);
+  const size_t o_shotu = ar.add((size_t)std::max<int64_t>(L.n_shots, 1) * 8);
+  const size_t o_samples = ar.add((size_t)std::max<int64_t>(L.n_shots, 1) * 8);
+  const size_t o_outcomes = ar.add((size_t)n_items * 8);
+  const size_t o_consumed = ar.add((size_t)n_items * 8);
+  const size_t o_err = ar.add((size_t)n_items * 4);
+  const size_t o_fb = ar.add(8);
+  if ((c->device_budget && c->resident_bytes + (int64_t)ar.size > c->device_budget) ||
+      c->resident_bytes + (int64_t)ar.size > c->device_memory)
+    fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
+  hipError_t e = hipMalloc(&ar.base, ar.size);
+  if (e != hipSuccess) {
+    ar.base = nullptr;
+    (void)hipGetLastError();
+    fail(QM_ERR_BUDGET, "HIP batch exceeds the free device memory.");
+  }
+  hipStream_t st = c->stream;
+  auto up = [&](size_t off, const void *src, size_t bytes) {
+    if (bytes) QM_HIP(hipMemcpyAsync(ar.at<char>(off), src, bytes, hipMemcpyHostToDevice, st));
+  };
+  up(o_items, L.items.data(), L.items.size() * sizeof(BatchItem));
+  up(o_passes, L.prog.passes.data(), L.prog.passes.size() * sizeof(DevPass));
+  up(o_ops, L.prog.ops.data(), L.prog.ops.size() * sizeof(TileOp));
+  up(o_segs, L.prog.segs.data(), L.prog.segs.size() * sizeof(BatchSeg));
+  if (L.pools) up(o_pools, L.pools, (size_t)(n_items * L.stride) * 8);
+  up(o_angles, L.angles.data(), L.angles.size() * 8);
+  up(o_groups, L.groups.data(), L.groups.size() * sizeof(PauliGroup));
+  if (L.n_shots) up(o_shotu, L.shot_u, (size_t)L.n_shots * 8);
+  QM_HIP(hipMemsetAsync(ar.at<char>(o_fb), 0, 8, st));
+
+  BatchArgs a{};
+  a.states = ar.at<void>(o_states);
+  a.slot_len = L.slot_len;
+  a.keep = L.keep ? 1 : 0;
+  a.slots = L.slots;
+  a.items = ar.at<BatchItem>(o_items);
+  a.n_items = n_items;
+  a.passes = ar.at<DevPass>(o_passes);
+  a.ops = ar.at<TileOp>(o_ops);
+  a.segs = ar.at<BatchSeg>(o_segs);
+  a.pools = L.pools ? ar.at<double>(o_pools) : nullptr;
+  a.stride = L.stride;
+  a.angles = L.angles.empty() ? nullptr : ar.at<double>(o_angles);
+  a.groups = ar.at<PauliGroup>(o_groups);
+  a.numer = ar.at<double>(o_numer);
+  a.mass = ar.at<double>(o_mass);
+  a.shot_u = ar.at<double>(o_shotu);
+  a.samples = ar.at<int64_t>(o_samples);
+  a.outcomes = ar.at<int64_t>(o_outcomes);
+  a.consumed = ar.at<int64_t>(o_consumed);
+  a.err = ar.at<int32_t>(o_err);
+  a.fallbacks = ar.at<unsigned long long>(o_fb);
+
+  int64_t grid = L.keep ? 0 : slots;
+  if (L.timed) QM_HIP(hipEventRecord(c->ev_start, st));
+  launch_batch(c, L.precision, L.shape, a, &grid);
+  if (L.timed) QM_HIP(hipEventRecord(c->ev_stop, st));
+
+  L.err.assign((size_t)n_items, 0);
+  L.numer.assign((size_t)L.n_numer, 0.0);
+  L.mass.assign((size_t)n_items, 0.0);
+  L.samples.assign((size_t)L.n_shots, 0);
+  L.outcomes.assign((size_t)n_items, 0);
+  L.consumed.assign((size_t)n_items, 0);
+  unsigned long long fb = 0;
+  auto down = [&](void *dst, size_t off, size_t bytes) {
+    if (bytes) QM_HIP(hipMemcpyAsync(dst, ar.at<char>(off), bytes, hipMemcpyDeviceToHost, st));
+  };
+  down(L.err.data(), o_err, L.err.size() * 4);
+  down(L.numer.data(), o_numer, L.numer.size() * 8);
+  down(L.mass.data(), o_mass, L.mass.size() * 8);
+  down(L.samples.data(), o_samples, L.samples.size() * 8);
+  down(L.outcomes.data(), o_outcomes, L.outcomes.size() * 8);
+  down(L.consumed.data(), o_consumed, L.consumed.size() * 8);
+  down(&fb, o_fb, 8);
+  if (L.keep && L.on_states) L.on_states(ar.at<void>(o_states), st);
+  QM_HIP(hipStreamSynchronize(st));
+  if (L.timed) {
+    float ms = 0;
+    QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    L.device_ms = ms;
+  }
+  L.fallbacks = (int64_t)fb;
 }
+
+namespace {
 
 void validate_terms(const qm_pauli *terms, int64_t n_terms, int n) {
   if (n_terms < 0 || (n_terms > 0 && !terms)) fail(QM_ERR_ARG, "terms are null");
@@ -100,28 +207,23 @@ struct BatchOut {
 void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes, int64_t *consumed,
                bool timed, BatchOut &out) {
   const int n = job.qubits;
-  const FusedShape shape = batch_shape(job.precision, n);
-  if (shape.K == 0) fail(QM_ERR_UNSUPPORTED, "Batched execution needs at least 8 qubits.");
-  BatchProgram prog = plan_batch(job.steps, n, shape);
-  Groups g = group_terms(job.terms, job.n_terms);
-  const uint64_t len = 1ull << n;
+  BatchLaunch L;
+  L.precision = job.precision;
+  L.shape = batch_shape(job.precision, n);
+  if (L.shape.K == 0) fail(QM_ERR_UNSUPPORTED, "Batched execution needs at least 8 qubits.");
+  L.prog = plan_batch(job.steps, n, L.shape);
+  TermGroups g = group_terms(job.terms, job.n_terms);
+  L.groups = g.groups;
   const int64_t state_bytes = amp_bytes(job.precision) << n;
-
-  // Workgroup slots: as many states as can be resident at once, within the budget.
-  int64_t slots = std::min<int64_t>(job.items, (int64_t)c->compute_units * 2);
-  const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
-  const int64_t room = limit - c->resident_bytes - (64ll << 20);
-  slots = std::min<int64_t>(slots, room / state_bytes);
-  if (slots < 1) fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
-
-  std::vector<BatchItem> items((size_t)job.items);
+  L.slot_len = (int64_t)1 << n;
+  L.items.resize((size_t)job.items);
   for (int64_t i = 0; i < job.items; i++) {
-    BatchItem &it = items[(size_t)i];
+    BatchItem &it = L.items[(size_t)i];
     it = BatchItem{};
-    it.len = len;
-    it.tiles = 1ull << (n - shape.K);
+    it.len = 1ull << n;
+    it.tiles = 1ull << (n - L.shape.K);
     it.seg_begin = 0;
-    it.seg_count = (int32_t)prog.segs.size();
+    it.seg_count = (int32_t)L.prog.segs.size();
     it.group_begin = 0;
     it.group_count = (int32_t)g.groups.size();
     it.numer_off = i * job.n_terms;
@@ -130,115 +232,39 @@ void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes,
     it.measure = job.measure ? 1 : 0;
     it.want_mass = job.want_mass ? 1 : 0;
   }
-
-  Arena ar;
-  const size_t o_states = ar.add((size_t)(slots * state_bytes));
-  const size_t o_items = ar.add(items.size() * sizeof(BatchItem));
-  const size_t o_passes = ar.add(prog.passes.size() * sizeof(DevPass));
-  const size_t o_ops = ar.add(std::max<size_t>(prog.ops.size(), 1) * sizeof(TileOp));
-  const size_t o_segs = ar.add(prog.segs.size() * sizeof(BatchSeg));
-  const size_t o_pools = ar.add(job.pools ? (size_t)(job.items * job.stride) * 8 : 0);
-  const size_t o_angles = ar.add(job.angles.size() * 8);
-  const size_t o_groups = ar.add(std::max<size_t>(g.groups.size(), 1) * sizeof(PauliGroup));
-  const size_t o_numer = ar.add((size_t)std::max<int64_t>(job.items * job.n_terms, 1) * 8);
-  const size_t o_mass = ar.add((size_t)job.items * 8);
-  const size_t o_shotu = ar.add((size_t)std::max<int64_t>(job.items * job.shots, 1) * 8);
-  const size_t o_samples = ar.add((size_t)std::max<int64_t>(job.items * job.shots, 1) * 8);
-  const size_t o_outcomes = ar.add((size_t)job.items * 8);
-  const size_t o_consumed = ar.add((size_t)job.items * 8);
-  const size_t o_err = ar.add((size_t)job.items * 4);
-  const size_t o_fb = ar.add(8);
-  if ((c->device_budget && c->resident_bytes + (int64_t)ar.size > c->device_budget) ||
-      c->resident_bytes + (int64_t)ar.size > c->device_memory)
-    fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
-  hipError_t e = hipMalloc(&ar.base, ar.size);
-  if (e != hipSuccess) {
-    ar.base = nullptr;
-    (void)hipGetLastError();
-    fail(QM_ERR_BUDGET, "HIP batch exceeds the free device memory.");
-  }
-  hipStream_t st = c->stream;
-  auto up = [&](size_t off, const void *src, size_t bytes) {
-    if (bytes) QM_HIP(hipMemcpyAsync(ar.at<char>(off), src, bytes, hipMemcpyHostToDevice, st));
-  };
-  up(o_items, items.data(), items.size() * sizeof(BatchItem));
-  up(o_passes, prog.passes.data(), prog.passes.size() * sizeof(DevPass));
-  up(o_ops, prog.ops.data(), prog.ops.size() * sizeof(TileOp));
-  up(o_segs, prog.segs.data(), prog.segs.size() * sizeof(BatchSeg));
-  if (job.pools) up(o_pools, job.pools, (size_t)(job.items * job.stride) * 8);
-  up(o_angles, job.angles.data(), job.angles.size() * 8);
-  up(o_groups, g.groups.data(), g.groups.size() * sizeof(PauliGroup));
-  if (job.shots) up(o_shotu, job.shot_u, (size_t)(job.items * job.shots) * 8);
-  QM_HIP(hipMemsetAsync(ar.at<char>(o_fb), 0, 8, st));
-
-  BatchArgs a{};
-  a.states = ar.at<void>(o_states);
-  a.slot_len = (int64_t)len;
-  a.keep = 0;
-  a.slots = job.slots;
-  a.items = ar.at<BatchItem>(o_items);
-  a.n_items = job.items;
-  a.passes = ar.at<DevPass>(o_passes);
-  a.ops = ar.at<TileOp>(o_ops);
-  a.segs = ar.at<BatchSeg>(o_segs);
-  a.pools = job.pools ? ar.at<double>(o_pools) : nullptr;
-  a.stride = job.stride;
-  a.angles = job.angles.empty() ? nullptr : ar.at<double>(o_angles);
-  a.groups = ar.at<PauliGroup>(o_groups);
-  a.numer = ar.at<double>(o_numer);
-  a.mass = ar.at<double>(o_mass);
-  a.shot_u = ar.at<double>(o_shotu);
-  a.samples = ar.at<int64_t>(o_samples);
-  a.outcomes = ar.at<int64_t>(o_outcomes);
-  a.consumed = ar.at<int64_t>(o_consumed);
-  a.err = ar.at<int32_t>(o_err);
-  a.fallbacks = ar.at<unsigned long long>(o_fb);
-
-  int64_t grid = slots;
-  if (timed) QM_HIP(hipEventRecord(c->ev_start, st));
-  launch_batch(c, job.precision, shape, a, &grid);
-  if (timed) QM_HIP(hipEventRecord(c->ev_stop, st));
-
-  std::vector<int32_t> err((size_t)job.items);
-  std::vector<double> numer_sorted((size_t)(job.items * job.n_terms));
-  out.mass.resize((size_t)job.items);
-  unsigned long long fb = 0;
-  auto down = [&](void *dst, size_t off, size_t bytes) {
-    if (bytes) QM_HIP(hipMemcpyAsync(dst, ar.at<char>(off), bytes, hipMemcpyDeviceToHost, st));
-  };
-  down(err.data(), o_err, err.size() * 4);
-  down(numer_sorted.data(), o_numer, numer_sorted.size() * 8);
-  down(out.mass.data(), o_mass, out.mass.size() * 8);
-  down(&fb, o_fb, 8);
-  if (samples && job.shots) down(samples, o_samples, (size_t)(job.items * job.shots) * 8);
-  if (outcomes && job.measure) down(outcomes, o_outcomes, (size_t)job.items * 8);
-  if (consumed) down(consumed, o_consumed, (size_t)job.items * 8);
-  QM_HIP(hipStreamSynchronize(st));
-  if (timed) {
-    float ms = 0;
-    QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
-    out.device_ms = ms;
-  }
+  L.n_numer = job.items * job.n_terms;
+  L.n_shots = job.items * job.shots;
+  L.pools = job.pools;
+  L.stride = job.stride;
+  L.angles = job.angles;
+  L.slots = job.slots;
+  L.shot_u = job.shot_u;
+  L.timed = timed;
+  run_batch_launch(c, L);
   for (int64_t i = 0; i < job.items; i++) {
-    const int code = err[(size_t)i];
+    const int code = L.err[(size_t)i];
     if (code == QM_ERR_POOL) fail(code, "The uniform pool is exhausted.");
     if (code == QM_ERR_MASS) fail(code, "The state has no finite positive probability mass.");
     if (code) fail(code, "batched state failed");
   }
-  out.numer.resize(numer_sorted.size());
+  if (samples && job.shots) std::memcpy(samples, L.samples.data(), L.samples.size() * 8);
+  if (outcomes && job.measure) std::memcpy(outcomes, L.outcomes.data(), L.outcomes.size() * 8);
+  if (consumed) std::memcpy(consumed, L.consumed.data(), L.consumed.size() * 8);
+  out.mass = L.mass;
+  out.numer.resize((size_t)(job.items * job.n_terms));
   for (int64_t i = 0; i < job.items; i++)
     for (int64_t k = 0; k < job.n_terms; k++)
-      out.numer[(size_t)(i * job.n_terms + g.order[(size_t)k])] =
-          numer_sorted[(size_t)(i * job.n_terms + k)];
-  out.fallbacks = (int64_t)fb;
-  out.passes = (int)prog.passes.size();
+      out.numer[(size_t)(i * job.n_terms + g.order[(size_t)k])] = L.numer[(size_t)(i * job.n_terms + k)];
+  out.fallbacks = L.fallbacks;
+  out.device_ms = L.device_ms;
+  out.passes = (int)L.prog.passes.size();
   out.state_bytes = state_bytes;
   // Each pass reads and writes the state; the final stage reads it once per
   // mass, per Pauli group and for the CDF.
   const double finals = (job.want_mass || job.n_terms ? 1.0 : 0.0) + (double)g.groups.size() +
                         (job.shots || job.measure ? 1.0 : 0.0);
   out.algorithmic_bytes =
-      (double)job.items * (double)state_bytes * (2.0 * (double)prog.passes.size() + finals);
+      (double)job.items * (double)state_bytes * (2.0 * (double)L.prog.passes.size() + finals);
 }
 
 void fill_profile(qm_profile *p, const BatchOut &o) {

@@ -483,14 +483,15 @@ template <typename T, int L, int R>
 void launch_batch_t(Ctx *c, const BatchArgs &args, int64_t *grid_out) {
   constexpr int K = L + R;
   const size_t lds = sizeof(T) << K;
-  static int resident = 0;  // workgroups per CU for this instance (one device per process)
-  if (resident == 0) {
+  // Workgroups per CU for this instance, set up once per process (thread-safe
+  // initialisation: several lanes may launch concurrently).
+  static const int resident = [&] {
     QM_HIP(hipFuncSetAttribute((const void *)k_batch<T, L, R>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     QM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<T, L, R>, 1 << L, lds));
-    resident = std::max(per_cu, 1);
-  }
+    return std::max(per_cu, 1);
+  }();
   int64_t blocks = std::min<int64_t>(args.n_items, (int64_t)c->compute_units * resident);
   if (!args.keep && *grid_out > 0) blocks = std::min<int64_t>(blocks, *grid_out);
   blocks = std::max<int64_t>(blocks, 1);

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
+#include <vector>
 
 #include "dev_reduce.hpp"
 #include "plan.hpp"
@@ -50,6 +52,47 @@ struct BatchArgs {
   int32_t *err;          // [items] status per item
   unsigned long long *fallbacks;  // margin fallbacks (sequential re-decisions)
 };
+
+// Pauli terms grouped by flip mask (stable in term order, at most
+// kTermsPerGroup per group); order[k] = original index of sorted term k.
+struct TermGroups {
+  std::vector<PauliGroup> groups;
+  std::vector<int64_t> order;
+};
+TermGroups group_terms(const qm_pauli *terms, int64_t n_terms);
+
+// Host description of one batched launch.  Inputs are set by the caller;
+// run_batch_launch uploads them in one arena, launches once, and fills the
+// outputs (per item status, numerators in group order, masses, samples,
+// outcomes, draw counts).  With `keep`, item states live at
+// items[i].state_off in one buffer of keep_len amplitudes and `on_states` is
+// called with it (on the context's stream) before the arena is released.
+struct BatchLaunch {
+  int precision = QM_C128;
+  FusedShape shape{};
+  BatchProgram prog;
+  std::vector<PauliGroup> groups;
+  std::vector<BatchItem> items;
+  int64_t slot_len = 0;
+  bool keep = false;
+  int64_t keep_len = 0;
+  int64_t n_numer = 0;
+  int64_t n_shots = 0;
+  const double *pools = nullptr;  // [items][stride]
+  int64_t stride = 0;
+  std::vector<double> angles;     // [items][slots][2]
+  int32_t slots = 0;
+  const double *shot_u = nullptr;  // [n_shots]
+  bool timed = false;
+  std::function<void(const void *states, hipStream_t stream)> on_states;
+  std::vector<int32_t> err;
+  std::vector<double> numer, mass;
+  std::vector<int64_t> samples, outcomes, consumed;
+  int64_t fallbacks = 0;
+  double device_ms = 0;
+};
+
+void run_batch_launch(Ctx *c, BatchLaunch &L);
 
 // Largest compiled batch tile shape with K <= qubits (K = 0: none).
 FusedShape batch_shape(int precision, int qubits);

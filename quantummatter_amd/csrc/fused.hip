@@ -84,14 +84,15 @@ static void launch_fused_t(State *st, const void *d_pass, const void *d_ops, hip
   constexpr int K = L + R;
   const uint64_t tiles = 1ull << (st->qubits - K);
   const size_t lds = sizeof(T) << K;
-  static int resident = 0;  // workgroups per CU for this instance (one device per process)
-  if (resident == 0) {
+  // Workgroups per CU for this instance, set up once per process (thread-safe
+  // initialisation: several lanes may launch concurrently).
+  static const int resident = [&] {
     QM_HIP(hipFuncSetAttribute((const void *)k_fused<T, L, R>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     QM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused<T, L, R>, 1 << L, lds));
-    resident = std::max(per_cu, 1);
-  }
+    return std::max(per_cu, 1);
+  }();
   // Persistent grid: every workgroup resident, each walking its share of tiles.
   const uint64_t want = (uint64_t)st->ctx->compute_units * resident;
   const int blocks = (int)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));

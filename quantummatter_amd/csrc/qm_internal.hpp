@@ -97,6 +97,18 @@ void use_device(Ctx *c);  // checks the context and makes its device current
 void check_cancel(const volatile int32_t *cancel);
 std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n);  // validated, identity dropped
 
+// Resident states and gate execution (api.hip).
+State *state_new(Ctx *c, int qubits, int precision);  // checks the context's budget
+void state_free(State *s);
+struct ExecStats {
+  int launches = 0;
+  int passes = 0;
+  double bytes = 0;
+};
+// Runs prepared gates on a state with the chosen schedule (synchronous).
+ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule,
+                    const volatile int32_t *cancel);
+
 // Scratch management on the context's stream.
 double *ctx_partials(Ctx *c, int64_t n);
 int64_t *ctx_ipartials(Ctx *c, int64_t n);

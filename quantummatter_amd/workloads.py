@@ -152,3 +152,38 @@ def noisy_layer_circuit(n: int, layers: int, seed: int, p_depol: float = 1e-3,
         for q in range(n):
             nc.ApplyNoise(damp, q)
     return nc
+
+
+def serving_requests(count: int, seed: int, rate: float = 0.0, min_qubits: int = 10,
+                     max_qubits: int = 28, min_depth: int = 10, max_depth: int = 100) -> List[dict]:
+    """The mixed request stream: width log-uniform over [min_qubits, max_qubits]
+    (floor of a continuous log-uniform draw), depth uniform over
+    [min_depth, max_depth] layers of random_layer_circuit, kind uniform over
+    {0 amplitudes, 1 shots, 2 expectation}; arrivals Poisson at `rate` per
+    second (all at time 0 when rate is 0).  Each request carries its own seed
+    for its circuit, its 1024 shot uniforms or its 64-term Hamiltonian."""
+    rng = SplitMix64(seed)
+    out = []
+    t = 0.0
+    lo, hi = math.log(min_qubits), math.log(max_qubits + 1)
+    for i in range(count):
+        n = int(math.floor(math.exp(lo + rng.next_double() * (hi - lo))))
+        n = min(max(n, min_qubits), max_qubits)
+        depth = min_depth + min(int(rng.next_double() * (max_depth - min_depth + 1)),
+                                max_depth - min_depth)
+        kind = min(int(rng.next_double() * 3), 2)
+        if rate > 0:
+            t += -math.log(1.0 - rng.next_double()) / rate
+        out.append({"index": i, "arrival": t, "qubits": n, "depth": depth, "kind": kind,
+                    "seed": pool_seed(seed, i)})
+    return out
+
+
+def serving_payload(request: dict):
+    """(circuit, shot uniforms or None, Pauli observables or None) of one request."""
+    n, seed = request["qubits"], request["seed"]
+    circuit = random_layer_circuit(n, request["depth"], seed=seed)
+    u = uniforms(seed ^ 0x5A5A, 1024) if request["kind"] == 1 else None
+    obs = ([t.Observable for t in random_hamiltonian(n, 64, seed ^ 0xA5A5).Terms]
+           if request["kind"] == 2 else None)
+    return circuit, u, obs
