@@ -544,18 +544,26 @@ void reduce_inner(State *a, State *b, double *re, double *im, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// CDF search (StateVectorSimulator.cs:44-90, StateVector.cs:131-147).
+// CDF search (StateVectorSimulator.cs:44-90, StateVector.cs:131-147), exact.
 //
-// Chunk sums, an exclusive scan of the chunk sums, then one thread per draw
-// binary-searches the chunk prefix and walks its chunk sequentially.  Both
-// the reference's left-to-right sum and this hierarchical sum are within
-// gamma_N * total of the exact prefix (N terms, all non-negative), so a draw
-// farther than delta = (5N + 8) * 2^-53 * total from both adjacent
-// cumulative values gets the reference's decision.  Draws inside that margin
-// are flagged and re-decided by a single-lane sequential pass in the
-// reference's order.
+// The reference's cumulative is the left-to-right double sum
+// S_i = fl(S_{i-1} + p_i).  This reproduces it bit for bit in parallel:
+// while S stays in one binade [2^e, 2^(e+1)), every S is a multiple of
+// u = 2^(e-52), so fl(S + p) = S + k*u with k = p/u rounded to nearest and a
+// tie (frac = 1/2) rounded to the even multiple, which depends only on the
+// parity of S/u.  Per block of 4096 amplitudes, one wave computes the block's
+// increment in units of u exactly with integer prefix sums, for both start
+// parities, assuming the binade predicted by an approximate (tree) prefix.
+// One wave then walks the blocks in order with the exact running S: a block
+// whose exact start has the predicted binade and whose end stays below
+// 2^(e+1) is one integer add; any other block (a binade crossing, S = 0,
+// subnormal sums) is summed sequentially by the wave, exactly as the
+// reference does.  All-zero blocks leave S unchanged.  Each draw then
+// binary-searches the exact block starts and walks one block sequentially
+// from its exact start.  No decision depends on a rounding margin.
 // ---------------------------------------------------------------------------
 constexpr int64_t kChunk = 4096;
+constexpr int kNoBinade = -2147483647 - 1;
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ s, uint64_t n,
@@ -594,153 +602,204 @@ __global__ __launch_bounds__(1024) void k_scan(const double *__restrict__ sums, 
   if (t == 1023) prefix[count] = sh[1023];
 }
 
+// Per block (one wave): the increment of S over the block in units of
+// u = 2^(e-52) for start parity 0 and 1, e = the predicted binade.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_search(const T *__restrict__ s, uint64_t n, uint64_t chunk,
-                                                   const double *__restrict__ prefix, int64_t nchunks,
-                                                   const double *__restrict__ draws, int64_t shots,
-                                                   long long last, double delta, long long *out,
-                                                   int *ambiguous) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= shots) return;
-  const double draw = draws[k];
-  const double total = prefix[nchunks];
-  if (draw >= total) {  // only by rounding of u*total: always re-decided
-    out[k] = last;
-    ambiguous[k] = 1;
+__global__ __launch_bounds__(kBlock) void k_block_delta(const T *__restrict__ s, uint64_t n,
+                                                        uint64_t chunk, int64_t blocks,
+                                                        const double *__restrict__ prefix,
+                                                        int *epred, long long *delta) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (b >= blocks) return;  // whole wave
+  const double a0 = prefix[b], a1 = prefix[b + 1];
+  int e = kNoBinade;
+  if (a0 >= 0x1p-1022 && isfinite(a1) && ilogb(a0) == ilogb(a1)) e = ilogb(a0);
+  if (e == kNoBinade) {
+    if (lane == 0) epred[b] = kNoBinade;
     return;
   }
-  // Last chunk whose prefix is <= draw.
-  int64_t lo = 0, hi = nchunks - 1;
+  const int shift = 52 - e;
+  const uint64_t begin = (uint64_t)b * chunk;
+  const uint64_t len = begin + chunk < n ? chunk : n - begin;
+  long long t0 = 0, t1 = 0;
+  int ok = 1;
+  for (uint64_t g = 0; g < len; g += 64) {
+    const double p = g + lane < len ? prob_of(s[begin + g + lane]) : 0.0;
+    const double q = ldexp(p, shift);  // exact: a power-of-two scaling
+    if (!(q < 0x1p62)) ok = 0;
+    const double kf = floor(q);
+    const double fr = q - kf;
+    const bool tie = fr == 0.5;
+    const long long k = ok ? (long long)kf + (fr > 0.5 ? 1 : 0) : 0;  // a tie counts its floor
+    long long inc = k;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long v = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += v;
+    }
+    const long long total = __shfl(inc, 63, 64);
+    unsigned long long ties = __ballot(tie);
+    long long x0 = 0, x1 = 0;  // extra units from ties rounded up, per start parity
+    while (ties) {
+      const int j = __ffsll((long long)ties) - 1;
+      ties &= ties - 1;
+      const long long kj = __shfl(k, j, 64);
+      const long long before = __shfl(inc, j, 64) - kj;
+      x0 += (0 + t0 + before + x0 + kj) & 1;
+      x1 += (1 + t1 + before + x1 + kj) & 1;
+    }
+    t0 += total + x0;
+    t1 += total + x1;
+  }
+  ok = __all(ok);
+  if (lane == 0) {
+    epred[b] = ok ? e : kNoBinade;
+    delta[2 * b] = t0;
+    delta[2 * b + 1] = t1;
+  }
+}
+
+// The reference's sequential sum over one block, one add per amplitude in
+// index order, starting from S (every lane holds the same S).
+template <typename T>
+__device__ double serial_block(const T *__restrict__ s, uint64_t begin, uint64_t len, double S,
+                               int lane) {
+  for (uint64_t g = 0; g < len; g += 256) {
+    double p[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const uint64_t off = g + (uint64_t)r * 64 + lane;
+      p[r] = off < len ? prob_of(s[begin + off]) : 0.0;  // S + 0 == S: padding is exact
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      for (int j = 0; j < 64; j++) S = S + __shfl(p[r], j, 64);
+  }
+  return S;
+}
+
+// Exact block starts S_b (sstart[0..blocks]), one wave walking the blocks.
+template <typename T>
+__global__ __launch_bounds__(64) void k_chain(const T *__restrict__ s, uint64_t n, uint64_t chunk,
+                                              int64_t blocks, const double *__restrict__ sums,
+                                              const int *__restrict__ epred,
+                                              const long long *__restrict__ delta, double *sstart) {
+  const int lane = threadIdx.x;
+  double S = 0;
+  for (int64_t g0 = 0; g0 < blocks; g0 += 64) {
+    const int64_t mine = g0 + lane;
+    const bool valid = mine < blocks;
+    const double msum = valid ? sums[mine] : 0.0;
+    const int me = valid ? epred[mine] : kNoBinade;
+    const long long d0 = valid ? delta[2 * mine] : 0, d1 = valid ? delta[2 * mine + 1] : 0;
+    double mystart = 0;
+    const int cnt = blocks - g0 < 64 ? (int)(blocks - g0) : 64;
+    for (int j = 0; j < cnt; j++) {
+      if (lane == j) mystart = S;
+      if (__shfl(msum, j, 64) == 0) continue;  // every amplitude of the block is zero
+      const int e = __shfl(me, j, 64);
+      const long long dj0 = __shfl(d0, j, 64), dj1 = __shfl(d1, j, 64);
+      bool done = false;
+      if (e != kNoBinade && S >= 0x1p-1022 && ilogb(S) == e) {
+        const double u = ldexp(1.0, e - 52);
+        const long long m = (long long)(S / u);  // exact: S is a multiple of u below 2^(e+1)
+        const long long t = m + ((m & 1) ? dj1 : dj0);
+        if (t < (1ll << 53)) {  // the block ends inside the binade
+          S = (double)t * u;
+          done = true;
+        }
+      }
+      if (!done) {
+        const uint64_t begin = (uint64_t)(g0 + j) * chunk;
+        S = serial_block(s, begin, begin + chunk < n ? chunk : n - begin, S, lane);
+      }
+    }
+    if (valid) sstart[mine] = mystart;
+  }
+  if (lane == 0) sstart[blocks] = S;
+}
+
+// One thread per draw: first index whose sequential cumulative exceeds u * total.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_exact_search(const T *__restrict__ s, uint64_t n,
+                                                         uint64_t chunk, int64_t blocks,
+                                                         const double *__restrict__ sstart,
+                                                         const double *__restrict__ u, int64_t shots,
+                                                         long long last, long long *out) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= shots) return;
+  const double total = sstart[blocks];
+  const double draw = u[k] * total;
+  if (!(draw < total)) {  // past the last cumulative: the last supported index
+    out[k] = last;
+    return;
+  }
+  int64_t lo = 0, hi = blocks - 1;
   while (lo < hi) {
     const int64_t mid = lo + (hi - lo + 1) / 2;
-    if (prefix[mid] <= draw)
+    if (sstart[mid] <= draw)
       lo = mid;
     else
       hi = mid - 1;
   }
   const uint64_t begin = (uint64_t)lo * chunk;
   const uint64_t end = begin + chunk < n ? begin + chunk : n;
-  double cum = prefix[lo];
-  long long found = -1;
-  double before = cum, after = cum;
+  double cum = sstart[lo];
+  long long found = last;
   for (uint64_t i = begin; i < end; i++) {
-    const double p = prob_of(s[i]);
-    before = cum;
-    cum += p;
+    cum = cum + prob_of(s[i]);
     if (draw < cum) {
       found = (long long)i;
-      after = cum;
       break;
     }
   }
-  if (found < 0) {
-    out[k] = -1;
-    ambiguous[k] = 1;
-    return;
-  }
   out[k] = found;
-  ambiguous[k] = (draw - before) <= delta || (after - draw) <= delta;
-}
-
-// Sequential resolution in the reference's order: total mass, then for each
-// flagged draw (ascending u) the first index whose running sum exceeds u*total.
-template <typename T>
-__global__ void k_serial_resolve(const T *__restrict__ s, uint64_t n, const double *__restrict__ u,
-                                 int64_t count, long long *out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double total = 0;
-  long long last = -1;
-  for (uint64_t i = 0; i < n; i++) {
-    const double p = prob_of(s[i]);
-    total += p;
-    if (p > 0) last = (long long)i;
-  }
-  double cum = 0;
-  uint64_t i = 0;
-  for (int64_t k = 0; k < count; k++) {
-    const double draw = u[k] * total;
-    long long found = last;
-    while (i < n) {
-      const double next = cum + prob_of(s[i]);
-      if (draw < next) {
-        found = (long long)i;
-        break;
-      }
-      cum = next;
-      i++;
-    }
-    out[k] = found;
-  }
 }
 
 int64_t cdf_search(State *s, const double *uniforms, int64_t shots, int64_t *outcomes, hipStream_t st,
                    double *total_out) {
   const uint64_t n = (uint64_t)s->length;
   const uint64_t chunk = std::min<uint64_t>((uint64_t)kChunk, n);
-  const int64_t nchunks = (int64_t)((n + chunk - 1) / chunk);
-  // Last supported index (and a first validation) before the scratch below is taken.
+  const int64_t blocks = (int64_t)((n + chunk - 1) / chunk);
+  // Last supported index and the mass check before the scratch below is taken.
   MassResult m = reduce_mass(s, st);
   if (!std::isfinite(m.total) || !(m.total > 0))
     fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
-  // Scratch: sums[nchunks], prefix[nchunks+1], draws[shots], out[shots], amb[shots].
-  const int64_t need = nchunks + (nchunks + 1) + shots + shots + (shots + 1) / 2 + 4;
+  // Scratch (doubles): sums[B], prefix[B+1], sstart[B+1], delta[2B], epred[B/2+1],
+  // draws[shots], out[shots].
+  const int64_t need = blocks + (blocks + 1) + (blocks + 1) + 2 * blocks + (blocks / 2 + 1) +
+                       shots + shots + 8;
   double *buf = ctx_partials(s->ctx, need);
-  double *sums = buf, *prefix = sums + nchunks, *draws = prefix + nchunks + 1;
+  double *sums = buf, *prefix = sums + blocks, *sstart = prefix + blocks + 1;
+  long long *delta = (long long *)(sstart + blocks + 1);
+  int *epred = (int *)(delta + 2 * blocks);
+  double *draws = (double *)(epred + 2 * (blocks / 2 + 1));
   long long *out = (long long *)(draws + shots);
-  int *amb = (int *)(out + shots);
-  if (s->precision == QM_C128)
-    k_chunk_sums<C128><<<(int)nchunks, kBlock, 0, st>>>((const C128 *)s->data, n, chunk, sums);
-  else
-    k_chunk_sums<C64><<<(int)nchunks, kBlock, 0, st>>>((const C64 *)s->data, n, chunk, sums);
-  QM_HIP(hipGetLastError());
-  k_scan<<<1, 1024, 0, st>>>(sums, nchunks, prefix);
+  const int dblocks = (int)((blocks + kBlock / 64 - 1) / (kBlock / 64));
+  const int sblocks = (int)((shots + kBlock - 1) / kBlock);
+  QM_HIP(hipMemcpyAsync(draws, uniforms, sizeof(double) * shots, hipMemcpyHostToDevice, st));
+#define QM_CDF(TT)                                                                                   \
+  k_chunk_sums<TT><<<(int)blocks, kBlock, 0, st>>>((const TT *)s->data, n, chunk, sums);            \
+  k_scan<<<1, 1024, 0, st>>>(sums, blocks, prefix);                                                \
+  k_block_delta<TT><<<dblocks, kBlock, 0, st>>>((const TT *)s->data, n, chunk, blocks, prefix,      \
+                                                epred, delta);                                       \
+  k_chain<TT><<<1, 64, 0, st>>>((const TT *)s->data, n, chunk, blocks, sums, epred, delta, sstart);  \
+  k_exact_search<TT><<<sblocks, kBlock, 0, st>>>((const TT *)s->data, n, chunk, blocks, sstart,    \
+                                                 draws, shots, (long long)m.last_supported, out);
+  if (s->precision == QM_C128) {
+    QM_CDF(C128)
+  } else {
+    QM_CDF(C64)
+  }
+#undef QM_CDF
   QM_HIP(hipGetLastError());
   double *pin = ctx_pinned(s->ctx, 1);
-  QM_HIP(hipMemcpyAsync(pin, prefix + nchunks, sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipMemcpyAsync(pin, sstart + blocks, sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipMemcpyAsync(outcomes, out, sizeof(long long) * shots, hipMemcpyDeviceToHost, st));
   QM_HIP(hipStreamSynchronize(st));
-  const double total = pin[0];
-  if (!std::isfinite(total) || !(total > 0))
-    fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
-  if (total_out) *total_out = total;
-  std::vector<double> d(shots);
-  for (int64_t k = 0; k < shots; k++) d[k] = uniforms[k] * total;
-  QM_HIP(hipMemcpyAsync(draws, d.data(), sizeof(double) * shots, hipMemcpyHostToDevice, st));
-  const double delta = (5.0 * (double)n + 8.0) * 0x1p-53 * total;
-  const int blocks = (int)((shots + kBlock - 1) / kBlock);
-  if (s->precision == QM_C128)
-    k_search<C128><<<blocks, kBlock, 0, st>>>((const C128 *)s->data, n, chunk, prefix, nchunks, draws,
-                                              shots, (long long)m.last_supported, delta, out, amb);
-  else
-    k_search<C64><<<blocks, kBlock, 0, st>>>((const C64 *)s->data, n, chunk, prefix, nchunks, draws,
-                                             shots, (long long)m.last_supported, delta, out, amb);
-  QM_HIP(hipGetLastError());
-  std::vector<long long> h_out(shots);
-  std::vector<int> h_amb(shots);
-  QM_HIP(hipMemcpyAsync(h_out.data(), out, sizeof(long long) * shots, hipMemcpyDeviceToHost, st));
-  QM_HIP(hipMemcpyAsync(h_amb.data(), amb, sizeof(int) * shots, hipMemcpyDeviceToHost, st));
-  QM_HIP(hipStreamSynchronize(st));
-  std::vector<int64_t> flagged;
-  for (int64_t k = 0; k < shots; k++) {
-    outcomes[k] = h_out[k];
-    if (h_amb[k]) flagged.push_back(k);
-  }
-  if (!flagged.empty()) {
-    std::sort(flagged.begin(), flagged.end(),
-              [&](int64_t a, int64_t b) { return uniforms[a] < uniforms[b]; });
-    std::vector<double> fu(flagged.size());
-    for (size_t i = 0; i < flagged.size(); i++) fu[i] = uniforms[flagged[i]];
-    const int64_t nf = (int64_t)flagged.size();
-    QM_HIP(hipMemcpyAsync(draws, fu.data(), sizeof(double) * nf, hipMemcpyHostToDevice, st));
-    if (s->precision == QM_C128)
-      k_serial_resolve<C128><<<1, 64, 0, st>>>((const C128 *)s->data, n, draws, nf, out);
-    else
-      k_serial_resolve<C64><<<1, 64, 0, st>>>((const C64 *)s->data, n, draws, nf, out);
-    QM_HIP(hipGetLastError());
-    QM_HIP(hipMemcpyAsync(h_out.data(), out, sizeof(long long) * nf, hipMemcpyDeviceToHost, st));
-    QM_HIP(hipStreamSynchronize(st));
-    for (int64_t i = 0; i < nf; i++) outcomes[flagged[i]] = h_out[i];
-  }
-  return (int64_t)flagged.size();
+  if (total_out) *total_out = pin[0];
+  return 0;
 }
 
 }  // namespace qm

@@ -215,3 +215,31 @@ def test_depolarizing_and_damping_branches_on_device():
                 cur = s.ApplyNoise(ch, 5, np.array([u]))
                 assert cur == used
                 assert np.max(np.abs(s.Snapshot() - want)) <= 1e-12
+
+
+@pytest.mark.parametrize("n,layers", [(22, 3), (24, 1)])
+def test_wide_sampling_is_exact(n, layers):
+    """At these widths the sequential-sum rounding exceeds single probabilities,
+    so only an exact emulation of the reference's cumulative gives its samples."""
+    c = workloads.random_layer_circuit(n, layers, seed=n)
+    psi = oracle.execute(n, c.to_ops())
+    u = workloads.uniforms(77 + n, 1024)
+    want = oracle.sample(n, psi, u)
+    with session(n, state=psi) as s:
+        got = s.Sample(u)
+        assert s.BoundaryFallbacks() == 0
+    assert np.array_equal(got, want)
+
+
+def test_uniform_superposition_sampling_is_exact():
+    """Equal probabilities 2^-n: every cumulative is exact, draws land on boundaries."""
+    n = 20
+    c = QuantumCircuit(n)
+    for q in range(n):
+        c.H(q)
+    psi = oracle.execute(n, c.to_ops())
+    u = np.concatenate([np.arange(0, 1024) / 1024.0, workloads.uniforms(5, 1024)])
+    want = oracle.sample(n, psi, u)
+    with session(n, state=psi) as s:
+        got = s.Sample(u)
+    assert np.array_equal(got, want)
