@@ -7,7 +7,10 @@ execution of the full 40-layer circuit on the resident state.  `value` is
 gate-layers per second over the timed steps (wall clock around K steps, each
 synchronised), with the state already in HBM.
 
-Also reported: the dominant kernel's roofline (the fused tile pass: 2 x 16 B x
+The passes run as compiled straight-line kernels (one per planned pass,
+compiled during the warmup and cached by circuit structure; compile time is
+reported in config), or with --schedule fused through the interpreting kernel.
+Also reported: the dominant kernel's roofline (the tile pass: 2 x 16 B x
 2^30 algorithmic bytes per launch, average duration from HIP events recorded
 by the library on its stream around the timed sequence), and the CPU baseline
 (the reference's parallel AVX-512 CPU path, restated in C++ in oracle/, timed
@@ -37,6 +40,10 @@ def parse():
     ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form extras")
+    ap.add_argument("--schedule", choices=["compiled", "fused"], default="compiled",
+                    help="compiled: fused passes as straight-line kernels (compiled during warmup, "
+                         "cached by circuit structure); fused: the interpreting tile-pass kernel")
     return ap.parse_args()
 
 
@@ -161,8 +168,11 @@ def main():
                                    "a batch across GPUs is not built"}))
         sys.exit(2)
     from quantummatter_amd import workloads
-    from quantummatter_amd.backend import (C64, C128, SCHED_FUSED, SCHED_PER_GATE, HipBackendOptions,
-                                           HipStateVectorBackend, plan_fused)
+    import ctypes as C
+    from quantummatter_amd._lib import lib
+    from quantummatter_amd.backend import (C64, C128, SCHED_COMPILED, SCHED_FUSED, SCHED_PER_GATE,
+                                           HipBackendOptions, HipStateVectorBackend, plan_fused)
+    sched = SCHED_COMPILED if args.schedule == "compiled" else SCHED_FUSED
     prec = C128 if args.precision == "c128" else C64
     n, L = args.qubits, args.layers
     circuit = workloads.random_layer_circuit(n, L, seed=1)
@@ -171,15 +181,19 @@ def main():
     amp_bytes = 16 if prec == C128 else 8
     with HipStateVectorBackend(HipBackendOptions(EnableProfiling=True)) as be:
         with be.CreateSession(n, prec) as s:
-            for _ in range(args.warmup):
+            t_warm = time.perf_counter()
+            for _ in range(max(args.warmup, 1 if sched == SCHED_COMPILED else 0)):
                 s.Reset()
-                s.EvolveOps(ops, SCHED_FUSED)
+                s.EvolveOps(ops, sched)
+            t_warm = time.perf_counter() - t_warm
+            compiled, hits, compile_s = C.c_int64(), C.c_int64(), C.c_double()
+            lib().qm_jit_stats(C.byref(compiled), C.byref(hits), C.byref(compile_s))
             device_ms = []
             t_total = 0.0
             for _ in range(args.steps):
                 s.Reset()
                 t0 = time.perf_counter()
-                prof = s.EvolveOps(ops, SCHED_FUSED, profile=True)
+                prof = s.EvolveOps(ops, sched, profile=True)
                 t_total += time.perf_counter() - t0
                 device_ms.append(prof.DeviceEvolutionMs)
             ms_per_step = t_total / args.steps * 1e3
@@ -188,6 +202,15 @@ def main():
             bytes_per_launch = 2.0 * amp_bytes * (1 << n)
             achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
             extras = {}
+            if not args.no_extras and sched == SCHED_COMPILED:
+                # The same passes through the interpreting kernel (no compilation).
+                s.Reset()
+                pf = s.EvolveOps(ops, SCHED_FUSED, profile=True)
+                extras["interpreted_fused_schedule"] = {
+                    "gate_layers_per_s": L / (pf.DeviceEvolutionMs * 1e-3),
+                    "ms_per_pass": pf.DeviceEvolutionMs / pf.Passes,
+                    "algorithmic_GBps": pf.AlgorithmicBytes / (pf.DeviceEvolutionMs * 1e-3) / 1e9,
+                }
             if not args.no_extras:
                 # The reference's GPU shape: one launch per gate, on two layers.
                 two = workloads.random_layer_circuit(n, 2, seed=1).to_ops()
@@ -198,9 +221,10 @@ def main():
                     "launches_per_layer": p2.KernelLaunches / 2,
                     "algorithmic_GBps": p2.AlgorithmicBytes / (p2.DeviceEvolutionMs * 1e-3) / 1e9,
                 }
-        if not args.no_extras:
+        if not args.no_extras and not args.no_batched:
             extras["batched"] = batched_extras(be)
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"fused_pass_{args.precision}_{n}q_pmc.txt"))
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles",
+                                       f"{args.schedule}_pass_{args.precision}_{n}q_pmc.txt"))
     line = {
         "metric": "30q c128 gate-layers/s + % HBM peak; served circuits/s with p50/p99 latency",
         "value": L * args.steps / t_total,
@@ -215,13 +239,16 @@ def main():
         "dtype": "f64" if prec == C128 else "f32",
         "data": "synthetic (seeded SplitMix64 random layer circuit, seed 1)",
         "config": {"workload": f"{n}q {args.precision} random layer circuit, {L} layers "
-                               f"({gates} gates), fused tile passes",
+                               f"({gates} gates), {args.schedule} tile passes",
                    "qubits": n, "layers": L, "gates": gates, "passes": passes,
                    "passes_per_layer": passes / L, "lds_transposes": transposes,
-                   "device_ms_per_step": dev_ms},
+                   "device_ms_per_step": dev_ms, "schedule": args.schedule,
+                   "pass_kernels_compiled": compiled.value, "compile_s": compile_s.value,
+                   "warmup_s": t_warm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_fused (tile pass)", "bytes_per_launch": bytes_per_launch,
+                     "kernel": "qm_pass (compiled tile pass)" if sched == SCHED_COMPILED
+                     else "k_fused (tile pass)", "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": launch_ms},
     }
     line.update(extras)

@@ -46,6 +46,8 @@ extern "C" {
 /* Execution schedules for qm_state_execute. */
 #define QM_SCHED_PER_GATE 0 /* one launch per non-identity gate (the reference's CUDA shape) */
 #define QM_SCHED_FUSED 1    /* register/LDS tile passes: many gates per HBM pass (default) */
+#define QM_SCHED_COMPILED 2 /* the same passes compiled to straight-line kernels (hipRTC), cached
+                               by circuit structure: the first run of a structure compiles */
 
 /* GateOperation (src/QuantumMatter.Core/QuantumCircuit.cs:49-61).  kind uses
  * the GateKind numbering (:4-40); control/second_control are -1 when absent;
@@ -214,6 +216,14 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
  * margin of a boundary were re-decided with the reference's sequential
  * summation order.  Returns the count accumulated on this state. */
 int qm_state_boundary_fallbacks(qm_state *state, int64_t *count);
+
+/* Pass kernels compiled for QM_SCHED_COMPILED so far in this process, cache
+ * hits, and the wall time spent compiling (no counterpart in the reference). */
+int qm_jit_stats(int64_t *compiled, int64_t *cache_hits, double *compile_seconds);
+/* Generates and compiles (without loading) the pass kernels of a plan for the
+ * given tile shape; *passes = kernels compiled.  Needs no device. */
+int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n_ops,
+                         int32_t tile_bits, int32_t lane_bits, int32_t *passes);
 
 /* ---------------------------------------------------------------------------
  * Batched execution: one program over many same-width states, one kernel

@@ -70,6 +70,8 @@ SIGNATURES = {
     "qm_trajectories_run": [P, I32, I32, P, I64, P, I64, I64, P, I64, I64, P, I32, P, P, P, P, P],
     "qm_energy_gradient_batch": [P, I32, I32, P, P, I64, P, I64, P, P, I64, P, P, P],
     "qm_energy_batch": [P, I32, I32, P, P, I64, P, I64, I64, P, P, I64, P, P],
+    "qm_jit_stats": [P, P, P],
+    "qm_jit_compile_check": [I32, I32, P, I64, I32, I32, P],
     "qm_server_create": [P, P],
     "qm_server_submit": [P, P, P],
     "qm_server_wait": [P, I64, P],

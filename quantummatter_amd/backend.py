@@ -27,6 +27,7 @@ C128 = 0
 C64 = 1
 SCHED_PER_GATE = 0
 SCHED_FUSED = 1
+SCHED_COMPILED = 2  # fused passes compiled to straight-line kernels, cached by structure
 
 
 @dataclass(frozen=True)

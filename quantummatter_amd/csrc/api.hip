@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "jit.hpp"
 #include "qm_internal.hpp"
 
 namespace qm {
@@ -110,12 +111,34 @@ std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
 
 
 // Runs prepared gates on a resident state with the chosen schedule.
-ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule,
-                           const volatile int32_t *cancel) {
+ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in,
+                    const volatile int32_t *cancel) {
   Ctx *c = s->ctx;
   ExecStats st;
   const double state_bytes = (double)s->bytes;
   const FusedShape shape = c->shape[s->precision];
+  int schedule = schedule_in;
+  if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
+    FusedPlan plan = plan_fused(gates, s->qubits, shape);
+    if (jit_supported(plan)) {
+      JitProgram jp = jit_prepare(plan, s->precision, c->device);
+      const int64_t bytes = (int64_t)(jp.coef.size() * sizeof(double));
+      double *d_coef = (double *)ctx_program(c, bytes + 256);
+      if (bytes)
+        QM_HIP(hipMemcpyAsync(d_coef, jp.coef.data(), (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+      for (size_t i = 0; i < plan.passes.size(); i++) {
+        check_cancel(cancel);
+        jit_launch(s, shape, jp, i, d_coef, c->stream);
+        st.launches++;
+        st.passes++;
+        st.bytes += 2 * state_bytes;
+        if (cancel && (st.launches & 31) == 0) QM_HIP(hipStreamSynchronize(c->stream));
+      }
+      QM_HIP(hipStreamSynchronize(c->stream));
+      return st;
+    }
+    schedule = QM_SCHED_FUSED;
+  }
   if (schedule == QM_SCHED_FUSED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     std::vector<TileOp> flat = plan.flat_ops();
@@ -472,7 +495,7 @@ int qm_state_execute(qm_state *h, const qm_op *ops, int64_t n_ops, int32_t sched
     Ctx *c = s->ctx;
     if (profile && !c->profiling)
       fail(QM_ERR_STATE, "EnableProfiling must be selected when creating the HIP backend.");
-    if (schedule != QM_SCHED_PER_GATE && schedule != QM_SCHED_FUSED)
+    if (schedule != QM_SCHED_PER_GATE && schedule != QM_SCHED_FUSED && schedule != QM_SCHED_COMPILED)
       fail(QM_ERR_ARG, "unknown schedule");
     check_cancel(cancel);
     std::lock_guard<std::recursive_mutex> g(c->lock);
@@ -650,6 +673,25 @@ int qm_state_inner(qm_state *a, qm_state *b, double *re, double *im) {
     std::lock_guard<std::recursive_mutex> g(x->ctx->lock);
     reduce_inner(x, y, re, im, x->ctx->stream);
   });
+}
+
+int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n_ops,
+                         int32_t tile_bits, int32_t lane_bits, int32_t *passes) {
+  return guarded([&] {
+    if (!passes) fail(QM_ERR_ARG, "passes is null");
+    if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+    const int R = tile_bits - lane_bits;
+    if (!fused_shape_supported(precision, lane_bits, R)) fail(QM_ERR_UNSUPPORTED, "tile shape is not compiled");
+    if (qubits < tile_bits || qubits > 34) fail(QM_ERR_RANGE, "qubits");
+    std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
+    FusedPlan plan = plan_fused(gates, qubits,
+                                FusedShape{tile_bits, lane_bits, R, precision == QM_C128 ? 3 : 4});
+    *passes = jit_compile_check(plan, precision);
+  });
+}
+
+int qm_jit_stats(int64_t *compiled, int64_t *cache_hits, double *compile_seconds) {
+  return guarded([&] { jit_stats(compiled, cache_hits, compile_seconds); });
 }
 
 int qm_state_boundary_fallbacks(qm_state *h, int64_t *count) {

@@ -1,0 +1,486 @@
+// Compiled fused passes: each planned tile pass becomes a straight-line HIP
+// kernel (hipRTC, gfx950), with every register index, control mask, LDS
+// address and load/store offset a compile-time constant and only the gate
+// coefficients read from memory.  This removes the per-op dispatch of the
+// interpreting kernel (fused.hip): no runtime switch over register bits, so
+// no register shuffling between ops, and register controls select at compile
+// time.  Kernels are cached by their source text (the pass structure without
+// coefficients), so re-running a circuit structure with new angles reuses them.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "jit.hpp"
+#include "qm_internal.hpp"
+
+namespace qm {
+namespace {
+
+// Device helpers of the generated kernels.  Every update writes each output
+// over the input it replaces after that input's last use, so the compiler
+// needs no copies; register controls are template masks.
+const char *kJitPrelude = R"QMJIT(
+struct __attribute__((aligned(16))) qj_c128 { double re, im; };
+struct __attribute__((aligned(8))) qj_c64 { float re, im; };
+__device__ __forceinline__ double qj_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float qj_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename F, int N, int B, int M>
+__device__ __forceinline__ void qj_rx(F (&vr)[N], F (&vi)[N], F c, F s) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & (1 << B)) || (r & M) != M) continue;
+    const int q = r | (1 << B);
+    const F a = c * vr[r], b = s * vr[r], e = c * vi[r], f = s * vi[r];
+    vr[r] = qj_fma(s, vi[q], a);
+    vi[q] = qj_fma(c, vi[q], -b);
+    vi[r] = qj_fma(-s, vr[q], e);
+    vr[q] = qj_fma(c, vr[q], f);
+  }
+}
+template <typename F, int N, int B, int M>
+__device__ __forceinline__ void qj_ry(F (&vr)[N], F (&vi)[N], F c, F s) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & (1 << B)) || (r & M) != M) continue;
+    const int q = r | (1 << B);
+    const F a = c * vr[r], b = s * vr[r], e = c * vi[r], f = s * vi[r];
+    vr[r] = qj_fma(-s, vr[q], a);
+    vr[q] = qj_fma(c, vr[q], b);
+    vi[r] = qj_fma(-s, vi[q], e);
+    vi[q] = qj_fma(c, vi[q], f);
+  }
+}
+template <typename F, int N, int B, int M>
+__device__ __forceinline__ void qj_h(F (&vr)[N], F (&vi)[N], F h) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & (1 << B)) || (r & M) != M) continue;
+    const int q = r | (1 << B);
+    const F a = h * vr[r], e = h * vi[r];
+    vr[r] = qj_fma(h, vr[q], a);
+    vr[q] = qj_fma(-h, vr[q], a);
+    vi[r] = qj_fma(h, vi[q], e);
+    vi[q] = qj_fma(-h, vi[q], e);
+  }
+}
+template <typename F, int N, int B, int M>
+__device__ __forceinline__ void qj_y(F (&vr)[N], F (&vi)[N]) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & (1 << B)) || (r & M) != M) continue;
+    const int q = r | (1 << B);
+    const F x0 = vr[r], x1 = vi[r];
+    vr[r] = vi[q];
+    vi[r] = -vr[q];
+    vr[q] = -x1;
+    vi[q] = x0;
+  }
+}
+template <typename F, int N, int B, int M>
+__device__ __forceinline__ void qj_x(F (&vr)[N], F (&vi)[N]) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & (1 << B)) || (r & M) != M) continue;
+    const int q = r | (1 << B);
+    const F a = vr[r], b = vi[r];
+    vr[r] = vr[q];
+    vi[r] = vi[q];
+    vr[q] = a;
+    vi[q] = b;
+  }
+}
+template <typename F, int N, int B, int M>
+__device__ __forceinline__ void qj_mat(F (&vr)[N], F (&vi)[N], const double *c) {
+  const F c0 = (F)c[0], c1 = (F)c[1], c2 = (F)c[2], c3 = (F)c[3];
+  const F c4 = (F)c[4], c5 = (F)c[5], c6 = (F)c[6], c7 = (F)c[7];
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & (1 << B)) || (r & M) != M) continue;
+    const int q = r | (1 << B);
+    const F x0 = vr[r], x1 = vi[r], y0 = vr[q], y1 = vi[q];
+    vr[r] = qj_fma(-c3, y1, qj_fma(c2, y0, qj_fma(-c1, x1, c0 * x0)));
+    vi[r] = qj_fma(c2, y1, qj_fma(c3, y0, qj_fma(c0, x1, c1 * x0)));
+    vr[q] = qj_fma(-c7, y1, qj_fma(c6, y0, qj_fma(-c5, x1, c4 * x0)));
+    vi[q] = qj_fma(c6, y1, qj_fma(c7, y0, qj_fma(c4, x1, c5 * x0)));
+  }
+}
+template <typename F>
+__device__ __forceinline__ void qj_mul(F &xr, F &xi, F pr, F pi) {
+  const F a = xi * pi, b = xr * pi;
+  xr = qj_fma(xr, pr, -a);
+  xi = qj_fma(xi, pr, b);
+}
+// diag(A, D) on register bit B (A skipped unless FULL), registers under mask M.
+template <typename F, int N, int B, int M, bool FULL>
+__device__ __forceinline__ void qj_diag(F (&vr)[N], F (&vi)[N], F ar, F ai, F dr, F di) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & M) != M) continue;
+    if (r & (1 << B))
+      qj_mul(vr[r], vi[r], dr, di);
+    else if (FULL)
+      qj_mul(vr[r], vi[r], ar, ai);
+  }
+}
+template <typename F, int N, int M>
+__device__ __forceinline__ void qj_phase(F (&vr)[N], F (&vi)[N], F pr, F pi) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if ((r & M) != M) continue;
+    qj_mul(vr[r], vi[r], pr, pi);
+  }
+}
+)QMJIT";
+
+std::string hex(uint64_t v) {
+  char b[32];
+  std::snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
+  return b;
+}
+
+// Source of one pass kernel, and the coefficient layout it reads.
+std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
+                        std::vector<double> &coef) {
+  const DevPass &d = p.dev;
+  const int L = sh.L, R = sh.R, N = 1 << R;
+  const bool c128 = precision == QM_C128;
+  const char *F = c128 ? "double" : "float";
+  const char *T = c128 ? "qj_c128" : "qj_c64";
+  std::ostringstream o;
+  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << L) << ", 2) void qm_pass(" << T
+    << " *__restrict__ s, const double *__restrict__ cf, unsigned long long tiles) {\n";
+  o << "  typedef " << F << " F;\n";
+  o << "  extern __shared__ " << T << " lds[];\n";
+  o << "  const unsigned t = threadIdx.x;\n";
+  auto lane_off = [&](const char *name, const uint64_t *offs) {
+    o << "  const unsigned long long " << name << " = 0ull";
+    for (int k = 0; k < L; k++) o << " | ((t & " << (1u << k) << "u) ? " << hex(offs[k]) << " : 0ull)";
+    o << ";\n";
+  };
+  lane_off("ll", d.load_lane_off);
+  lane_off("sl", d.store_lane_off);
+  // Lane part of the swizzled LDS address of every mapping a transpose touches.
+  std::vector<int> used;
+  auto use = [&](int slot) {
+    if (std::find(used.begin(), used.end(), slot) == used.end()) used.push_back(slot);
+  };
+  int cur = d.load_direct ? 0 : kIo;
+  for (int ph = 0; ph < d.nphases; ph++) {
+    if (cur != ph) use(cur), use(ph);
+    cur = ph;
+  }
+  if (!d.store_direct) use(cur), use(kIo);
+  for (int slot : used) {
+    o << "  const unsigned j" << slot << " = 0u";
+    for (int k = 0; k < L; k++) o << " ^ ((t & " << (1u << k) << "u) ? " << d.lane_swz[slot][k] << "u : 0u)";
+    o << ";\n";
+  }
+  o << "  for (unsigned long long tau = blockIdx.x; tau < tiles; tau += gridDim.x) {\n";
+  o << "    unsigned long long base = 0ull";
+  for (int k = 0; k < d.nruns; k++)
+    o << " | (((tau >> " << (int)d.run_src[k] << ") & " << hex((1ull << d.run_len[k]) - 1) << ") << "
+      << (int)d.run_dst[k] << ")";
+  o << ";\n";
+  o << "    F vr[" << N << "], vi[" << N << "];\n";
+  o << "    { const " << T << " *src = s + (base | ll);\n";
+  for (int r = 0; r < N; r++)
+    o << "      { const " << T << " a = src[" << hex(d.load_reg_off[r]) << "]; vr[" << r << "] = a.re; vi["
+      << r << "] = a.im; }\n";
+  o << "    }\n";
+  auto transpose = [&](int from, int to) {
+    o << "    __syncthreads();\n";
+    for (int r = 0; r < N; r++)
+      o << "    { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; lds[j" << from << " ^ "
+        << d.regswz[from][r] << "u] = a; }\n";
+    o << "    __syncthreads();\n";
+    for (int r = 0; r < N; r++)
+      o << "    { const " << T << " a = lds[j" << to << " ^ " << d.regswz[to][r] << "u]; vr[" << r
+        << "] = a.re; vi[" << r << "] = a.im; }\n";
+  };
+  auto cond = [&](const TileOp &op) {
+    std::string c;
+    if (op.lane_cmask) c += "(t & " + std::to_string(op.lane_cmask) + "u) == " + std::to_string(op.lane_cmask) + "u";
+    if (op.out_cmask) {
+      if (!c.empty()) c += " && ";
+      c += "(base & " + hex(op.out_cmask) + ") == " + hex(op.out_cmask);
+    }
+    return c;
+  };
+  auto cfix = [&](const double *v, int n) {
+    const size_t at = coef.size();
+    coef.insert(coef.end(), v, v + n);
+    return at;
+  };
+  const std::string tmpl = std::string("<F, ") + std::to_string(N) + ", ";
+  cur = d.load_direct ? 0 : kIo;
+  const TileOp *ops = p.ops.data();
+  for (int ph = 0; ph < d.nphases; ph++) {
+    if (cur != ph) transpose(cur, ph);
+    cur = ph;
+    for (int i = 0; i < d.phase_op_count[ph]; i++) {
+      const TileOp &op = ops[d.phase_op_begin[ph] + i];
+      const std::string c = cond(op);
+      o << "    " << (c.empty() ? "{" : "if (" + c + ") {") << " ";
+      const std::string m = std::to_string(op.reg_cmask);
+      if (op.rbit >= 0) {
+        const std::string bm = std::to_string(op.rbit) + ", " + m + ">";
+        switch (op.kind) {
+          case G_RX: {
+            const size_t k = cfix(op.c, 2);
+            o << "qj_rx" << tmpl << bm << "(vr, vi, (F)cf[" << k << "], (F)cf[" << k + 1 << "]);";
+            break;
+          }
+          case G_RY: {
+            const size_t k = cfix(op.c, 2);
+            o << "qj_ry" << tmpl << bm << "(vr, vi, (F)cf[" << k << "], (F)cf[" << k + 1 << "]);";
+            break;
+          }
+          case G_H: {
+            const size_t k = cfix(op.c, 1);
+            o << "qj_h" << tmpl << bm << "(vr, vi, (F)cf[" << k << "]);";
+            break;
+          }
+          case G_Y:
+            o << "qj_y" << tmpl << bm << "(vr, vi);";
+            break;
+          case G_X:
+            o << "qj_x" << tmpl << bm << "(vr, vi);";
+            break;
+          default: {  // G_MAT
+            const size_t k = cfix(op.c, 8);
+            o << "qj_mat" << tmpl << bm << "(vr, vi, cf + " << k << ");";
+            break;
+          }
+        }
+      } else {
+        const bool full = op.kind == G_DIAG;
+        const size_t k = cfix(op.c, 8);
+        const std::string A = "(F)cf[" + std::to_string(k) + "], (F)cf[" + std::to_string(k + 1) + "]";
+        const std::string D = "(F)cf[" + std::to_string(k + 6) + "], (F)cf[" + std::to_string(k + 7) + "]";
+        if (op.dreg) {
+          o << "qj_diag" << tmpl << __builtin_ctz(op.dreg) << ", " << m << ", " << (full ? "true" : "false")
+            << ">(vr, vi, " << A << ", " << D << ");";
+        } else {
+          std::string hi;
+          if (op.dlane) hi = "(t & " + std::to_string(op.dlane) + "u) != 0u";
+          if (op.dout) hi += std::string(hi.empty() ? "" : " || ") + "(base & " + hex(op.dout) + ") != 0ull";
+          if (!full) {
+            o << "if (" << hi << ") qj_phase" << tmpl << m << ">(vr, vi, " << D << ");";
+          } else {
+            o << "const bool hi = " << hi << "; qj_phase" << tmpl << m << ">(vr, vi, hi ? (F)cf[" << k + 6
+              << "] : (F)cf[" << k << "], hi ? (F)cf[" << k + 7 << "] : (F)cf[" << k + 1 << "]);";
+          }
+        }
+      }
+      o << " }\n";
+    }
+    const int dcnt = d.phase_diag_count[ph];
+    if (dcnt) {
+      // Merged lane/outside diagonals: one per-lane phase, one multiplication.
+      o << "    { F pr = 1, pi = 0;\n";
+      for (int i = 0; i < dcnt; i++) {
+        const TileOp &q = ops[d.phase_diag_begin[ph] + i];
+        const size_t k = cfix(q.c, 8);
+        const bool full = q.kind == G_DIAG;
+        std::string on = "true";
+        if (q.lane_cmask || q.out_cmask) on = cond(q);
+        std::string hi = "false";
+        if (q.dlane || q.dout) {
+          hi.clear();
+          if (q.dlane) hi = "(t & " + std::to_string(q.dlane) + "u) != 0u";
+          if (q.dout) hi += std::string(hi.empty() ? "" : " || ") + "(base & " + hex(q.dout) + ") != 0ull";
+        }
+        o << "      { const bool on = " << on << "; const bool hi = " << hi << ";\n";
+        o << "        const F fr = on ? (hi ? (F)cf[" << k + 6 << "] : " << (full ? "(F)cf[" + std::to_string(k) + "]" : "(F)1")
+          << ") : (F)1;\n";
+        o << "        const F fi = on ? (hi ? (F)cf[" << k + 7 << "] : " << (full ? "(F)cf[" + std::to_string(k + 1) + "]" : "(F)0")
+          << ") : (F)0;\n";
+        o << "        const F nr = pr * fr - pi * fi, ni = pr * fi + pi * fr; pr = nr; pi = ni; }\n";
+      }
+      o << "      qj_phase" << tmpl << "0>(vr, vi, pr, pi); }\n";
+    }
+  }
+  if (!d.store_direct) transpose(cur, kIo);
+  o << "    { " << T << " *dst = s + (base | sl);\n";
+  for (int r = 0; r < N; r++)
+    o << "      { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; dst[" << hex(d.store_reg_off[r])
+      << "] = a; }\n";
+  o << "    }\n  }\n}\n";
+  return o.str();
+}
+
+struct Compiled {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+struct JitCache {
+  std::mutex mu;
+  std::unordered_map<std::string, Compiled> map;  // key: device + source
+  std::atomic<int64_t> compiled{0}, hits{0};
+  std::atomic<int64_t> compile_us{0};
+};
+
+JitCache &cache() {
+  static JitCache c;
+  return c;
+}
+
+// hipRTC to a code object; empty result with the log in `err` on failure.
+std::vector<char> compile_one(const std::string &src, std::string &err) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "qm_pass.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    err = "hiprtcCreateProgram failed";
+    return {};
+  }
+  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-Xclang", "-target-feature",
+                        "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  std::vector<char> code;
+  if (rc != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    err = std::string(hiprtcGetErrorString(rc)) + ": " + log.substr(0, 4000);
+  } else {
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    code.resize(n);
+    hiprtcGetCode(prog, code.data());
+  }
+  hiprtcDestroyProgram(&prog);
+  return code;
+}
+
+}  // namespace
+
+bool jit_supported(const FusedPlan &plan) {
+  for (const FusedPass &p : plan.passes)
+    for (const TileOp &op : p.ops)
+      if (op.kind >= G_PHASEPROD) return false;
+  return true;
+}
+
+JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
+  JitProgram jp;
+  std::vector<std::string> keys(plan.passes.size());
+  for (size_t i = 0; i < plan.passes.size(); i++) {
+    // Coefficient indices in the source are relative to the pass (the source,
+    // and so the cache key, does not depend on where the pass sits).
+    std::vector<double> local;
+    std::string src = pass_source(plan.passes[i], plan.shape, precision, local);
+    jp.coef_offset.push_back((int64_t)jp.coef.size());
+    jp.coef.insert(jp.coef.end(), local.begin(), local.end());
+    keys[i] = std::to_string(device) + "\n" + kJitPrelude + src;
+  }
+  JitCache &c = cache();
+  std::vector<size_t> missing;
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    std::unordered_map<std::string, int> seen;
+    for (size_t i = 0; i < keys.size(); i++) {
+      if (c.map.count(keys[i])) {
+        c.hits++;
+      } else if (!seen.count(keys[i])) {
+        seen[keys[i]] = 1;
+        missing.push_back(i);
+      }
+    }
+  }
+  if (!missing.empty()) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::vector<char>> codes(missing.size());
+    std::vector<std::string> errs(missing.size());
+    std::atomic<size_t> next{0};
+    // hipRTC compiles scale with threads; 16 workers (the host share of one GPU).
+    const size_t workers = std::min<size_t>(missing.size(), 16);
+    std::vector<std::thread> th;
+    for (size_t w = 0; w < workers; w++)
+      th.emplace_back([&] {
+        for (size_t k; (k = next++) < missing.size();) {
+          const std::string &key = keys[missing[k]];
+          codes[k] = compile_one(key.substr(key.find('\n') + 1), errs[k]);
+        }
+      });
+    for (auto &x : th) x.join();
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (size_t k = 0; k < missing.size(); k++) {
+      if (codes[k].empty()) fail(QM_ERR_DEVICE, "pass compilation failed: " + errs[k]);
+      Compiled cp;
+      QM_HIP(hipModuleLoadData(&cp.mod, codes[k].data()));
+      QM_HIP(hipModuleGetFunction(&cp.fn, cp.mod, "qm_pass"));
+      c.map[keys[missing[k]]] = cp;
+      c.compiled++;
+    }
+    c.compile_us += (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                        std::chrono::steady_clock::now() - t0)
+                        .count();
+  }
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (const std::string &k : keys) jp.fns.push_back((void *)c.map[k].fn);
+  return jp;
+}
+
+void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,
+                const double *d_coef, hipStream_t st) {
+  const uint64_t tiles = 1ull << (s->qubits - shape.K);
+  const size_t lds = (size_t)amp_bytes(s->precision) << shape.K;
+  const uint64_t want = (uint64_t)s->ctx->compute_units * 2;
+  const unsigned blocks = (unsigned)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
+  void *state = s->data;
+  const double *cf = d_coef + jp.coef_offset[pass];
+  unsigned long long nt = tiles;
+  void *args[] = {&state, &cf, &nt};
+  QM_HIP(hipModuleLaunchKernel((hipFunction_t)jp.fns[pass], blocks, 1, 1, 1u << shape.L, 1, 1,
+                               (unsigned)lds, st, args, nullptr));
+}
+
+int jit_compile_check(const FusedPlan &plan, int precision) {
+  const char *dump = std::getenv("QM_JIT_DUMP");  // debugging aid: directory for the sources
+  int index = 0;
+  for (const FusedPass &p : plan.passes) {
+    std::string err;
+    std::vector<double> coef;
+    const std::string src = pass_source(p, plan.shape, precision, coef);
+    if (dump) {
+      FILE *f = std::fopen((std::string(dump) + "/pass" + std::to_string(index) + ".hip").c_str(), "w");
+      if (f) {
+        std::fputs(src.c_str(), f);
+        std::fclose(f);
+      }
+      f = std::fopen((std::string(dump) + "/pass" + std::to_string(index) + ".coef").c_str(), "w");
+      if (f) {
+        for (double v : coef) std::fprintf(f, "%.17g\n", v);
+        std::fclose(f);
+      }
+      index++;
+    }
+    const std::vector<char> code = compile_one(std::string(kJitPrelude) + src, err);
+    if (code.empty()) fail(QM_ERR_DEVICE, "pass compilation failed: " + err);
+  }
+  return (int)plan.passes.size();
+}
+
+void jit_stats(int64_t *compiled, int64_t *hits, double *seconds) {
+  JitCache &c = cache();
+  if (compiled) *compiled = c.compiled.load();
+  if (hits) *hits = c.hits.load();
+  if (seconds) *seconds = (double)c.compile_us.load() * 1e-6;
+}
+
+}  // namespace qm

@@ -1,0 +1,30 @@
+// Compiled fused passes (jit.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace qm {
+
+struct State;
+
+// Kernels of a plan's passes (cached by structure) and the coefficients they read.
+struct JitProgram {
+  std::vector<void *> fns;             // hipFunction_t per pass
+  std::vector<double> coef;            // all passes' coefficients
+  std::vector<int64_t> coef_offset;    // first coefficient of each pass
+};
+
+bool jit_supported(const FusedPlan &plan);  // no run-time resolved ops
+JitProgram jit_prepare(const FusedPlan &plan, int precision, int device);
+void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,
+                const double *d_coef, hipStream_t st);
+void jit_stats(int64_t *compiled, int64_t *hits, double *seconds);
+// Generates and compiles every pass kernel without loading it (no device needed).
+int jit_compile_check(const FusedPlan &plan, int precision);
+
+}  // namespace qm

@@ -205,4 +205,27 @@ __device__ __forceinline__ void lds_transpose(T *__restrict__ lds, const DevPass
 }
 
 
+// The same exchange through an LDS buffer of 2^K reals (half the bytes): real
+// parts first, then imaginary parts.  Halving the LDS footprint lets more
+// workgroups share a CU.
+template <typename F, int L, int REGS>
+__device__ __forceinline__ void lds_transpose_split(F *__restrict__ lds, const DevPass &d, int from,
+                                                    int to, uint32_t t, F (&vr)[REGS],
+                                                    F (&vi)[REGS]) {
+  const uint32_t jf = lane_swizzled<L>(d, from, t);
+  const uint32_t jt = lane_swizzled<L>(d, to, t);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < REGS; r++) lds[jf ^ d.regswz[from][r]] = vr[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < REGS; r++) vr[r] = lds[jt ^ d.regswz[to][r]];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < REGS; r++) lds[jf ^ d.regswz[from][r]] = vi[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < REGS; r++) vi[r] = lds[jt ^ d.regswz[to][r]];
+}
+
 }  // namespace qm
