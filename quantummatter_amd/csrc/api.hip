@@ -116,7 +116,7 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
   Ctx *c = s->ctx;
   ExecStats st;
   const double state_bytes = (double)s->bytes;
-  const FusedShape shape = c->shape[s->precision];
+  FusedShape shape = c->shape[s->precision];
   int schedule = schedule_in;
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
@@ -139,6 +139,8 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
     }
     schedule = QM_SCHED_FUSED;
   }
+  if (schedule == QM_SCHED_FUSED && !fused_shape_supported(s->precision, shape.L, shape.R))
+    shape = default_fused_shape(s->precision);
   if (schedule == QM_SCHED_FUSED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     std::vector<TileOp> flat = plan.flat_ops();
@@ -708,8 +710,13 @@ int qm_ctx_set_fused_shape(qm_ctx *h, int32_t precision, int32_t tile_bits, int3
     use_device(c);
     if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
     const int R = tile_bits - lane_bits;
-    if (!fused_shape_supported(precision, lane_bits, R))
-      fail(QM_ERR_UNSUPPORTED, "fused tile shape is not compiled");
+    // Any shape the compiled schedule can generate (its LDS exchange buffer holds
+    // 2^K reals within 64 KiB); the interpreting kernel uses this shape when it is
+    // one of its instantiations and its default shape otherwise.
+    const int max_k = precision == QM_C128 ? 13 : 14;
+    if (!fused_shape_supported(precision, lane_bits, R) &&
+        !(lane_bits >= 6 && lane_bits <= 10 && R >= 2 && R <= 5 && tile_bits <= max_k))
+      fail(QM_ERR_UNSUPPORTED, "fused tile shape is not supported");
     std::lock_guard<std::recursive_mutex> g(c->lock);
     c->shape[precision] = FusedShape{tile_bits, lane_bits, R, precision == QM_C128 ? 3 : 4};
   });

@@ -153,18 +153,43 @@ std::string hex(uint64_t v) {
 }
 
 // Source of one pass kernel, and the coefficient layout it reads.
+// Code-generation variant (tuning knob QM_JIT_VARIANT, default 1): bit 0
+// exchanges through an LDS buffer of reals (half the LDS, real then imaginary
+// parts, so more workgroups fit a CU), bit 2 forms the lane part of each LDS
+// address at its transpose instead of once per kernel.
+int jit_variant() {
+  static const int v = [] {
+    const char *e = std::getenv("QM_JIT_VARIANT");
+    return e ? std::atoi(e) : 1;
+  }();
+  return v;
+}
+
+size_t jit_lds_bytes(int precision, int K) {
+  const bool split = (jit_variant() & 1) != 0;
+  return (split ? (size_t)amp_bytes(precision) / 2 : (size_t)amp_bytes(precision)) << K;
+}
+
 std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
                         std::vector<double> &coef) {
+  const int variant = jit_variant();
+  const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
   const char *F = c128 ? "double" : "float";
   const char *T = c128 ? "qj_c128" : "qj_c64";
   std::ostringstream o;
-  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << L) << ", 2) void qm_pass(" << T
+  // QM_OCC (workgroups per CU the register allocation must allow) is defined
+  // when the kernel is compiled.
+  (void)occ3;
+  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << L) << ", QM_OCC) void qm_pass(" << T
     << " *__restrict__ s, const double *__restrict__ cf, unsigned long long tiles) {\n";
   o << "  typedef " << F << " F;\n";
-  o << "  extern __shared__ " << T << " lds[];\n";
+  if (split)
+    o << "  extern __shared__ F lds[];\n";
+  else
+    o << "  extern __shared__ " << T << " lds[];\n";
   o << "  const unsigned t = threadIdx.x;\n";
   auto lane_off = [&](const char *name, const uint64_t *offs) {
     o << "  const unsigned long long " << name << " = 0ull";
@@ -184,11 +209,14 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     cur = ph;
   }
   if (!d.store_direct) use(cur), use(kIo);
-  for (int slot : used) {
-    o << "  const unsigned j" << slot << " = 0u";
-    for (int k = 0; k < L; k++) o << " ^ ((t & " << (1u << k) << "u) ? " << d.lane_swz[slot][k] << "u : 0u)";
-    o << ";\n";
-  }
+  auto lane_addr = [&](int slot) {
+    std::string e = "(0u";
+    for (int k = 0; k < L; k++)
+      e += " ^ ((t & " + std::to_string(1u << k) + "u) ? " + std::to_string(d.lane_swz[slot][k]) + "u : 0u)";
+    return e + ")";
+  };
+  if (!inline_j)
+    for (int slot : used) o << "  const unsigned j" << slot << " = " << lane_addr(slot) << ";\n";
   o << "  for (unsigned long long tau = blockIdx.x; tau < tiles; tau += gridDim.x) {\n";
   o << "    unsigned long long base = 0ull";
   for (int k = 0; k < d.nruns; k++)
@@ -202,14 +230,31 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       << r << "] = a.im; }\n";
   o << "    }\n";
   auto transpose = [&](int from, int to) {
-    o << "    __syncthreads();\n";
-    for (int r = 0; r < N; r++)
-      o << "    { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; lds[j" << from << " ^ "
-        << d.regswz[from][r] << "u] = a; }\n";
-    o << "    __syncthreads();\n";
-    for (int r = 0; r < N; r++)
-      o << "    { const " << T << " a = lds[j" << to << " ^ " << d.regswz[to][r] << "u]; vr[" << r
-        << "] = a.re; vi[" << r << "] = a.im; }\n";
+    o << "    {\n";
+    if (inline_j) {
+      o << "    const unsigned j" << from << " = " << lane_addr(from) << ";\n";
+      o << "    const unsigned j" << to << " = " << lane_addr(to) << ";\n";
+    }
+    if (split) {
+      for (const char *part : {"vr", "vi"}) {
+        o << "    __syncthreads();\n";
+        for (int r = 0; r < N; r++)
+          o << "    lds[j" << from << " ^ " << d.regswz[from][r] << "u] = " << part << "[" << r << "];\n";
+        o << "    __syncthreads();\n";
+        for (int r = 0; r < N; r++)
+          o << "    " << part << "[" << r << "] = lds[j" << to << " ^ " << d.regswz[to][r] << "u];\n";
+      }
+    } else {
+      o << "    __syncthreads();\n";
+      for (int r = 0; r < N; r++)
+        o << "    { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; lds[j" << from << " ^ "
+          << d.regswz[from][r] << "u] = a; }\n";
+      o << "    __syncthreads();\n";
+      for (int r = 0; r < N; r++)
+        o << "    { const " << T << " a = lds[j" << to << " ^ " << d.regswz[to][r] << "u]; vr[" << r
+          << "] = a.re; vi[" << r << "] = a.im; }\n";
+    }
+    o << "    }\n";
   };
   auto cond = [&](const TileOp &op) {
     std::string c;
@@ -326,6 +371,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
 struct Compiled {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
+  int resident = 2;  // workgroups per CU
 };
 
 struct JitCache {
@@ -367,6 +413,23 @@ std::vector<char> compile_one(const std::string &src, std::string &err) {
   return code;
 }
 
+// Compiles sources on 16 worker threads (hipRTC compiles scale with threads).
+std::vector<std::vector<char>> compile_all(const std::vector<std::string> &srcs) {
+  std::vector<std::vector<char>> codes(srcs.size());
+  std::vector<std::string> errs(srcs.size());
+  std::atomic<size_t> next{0};
+  const size_t workers = std::min<size_t>(srcs.size(), 16);
+  std::vector<std::thread> th;
+  for (size_t w = 0; w < workers; w++)
+    th.emplace_back([&] {
+      for (size_t k; (k = next++) < srcs.size();) codes[k] = compile_one(srcs[k], errs[k]);
+    });
+  for (auto &x : th) x.join();
+  for (size_t k = 0; k < srcs.size(); k++)
+    if (codes[k].empty()) fail(QM_ERR_DEVICE, "pass compilation failed: " + errs[k]);
+  return codes;
+}
+
 }  // namespace
 
 bool jit_supported(const FusedPlan &plan) {
@@ -404,27 +467,41 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
   }
   if (!missing.empty()) {
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<std::vector<char>> codes(missing.size());
-    std::vector<std::string> errs(missing.size());
-    std::atomic<size_t> next{0};
-    // hipRTC compiles scale with threads; 16 workers (the host share of one GPU).
-    const size_t workers = std::min<size_t>(missing.size(), 16);
-    std::vector<std::thread> th;
-    for (size_t w = 0; w < workers; w++)
-      th.emplace_back([&] {
-        for (size_t k; (k = next++) < missing.size();) {
-          const std::string &key = keys[missing[k]];
-          codes[k] = compile_one(key.substr(key.find('\n') + 1), errs[k]);
+    // Highest occupancy (3, then 2 workgroups per CU) whose register
+    // allocation does not spill, per pass.
+    std::vector<size_t> todo(missing.size());
+    for (size_t k = 0; k < todo.size(); k++) todo[k] = k;
+    std::vector<Compiled> done(missing.size());
+    for (int occ = 3; occ >= 2 && !todo.empty(); occ--) {
+      std::vector<std::string> srcs;
+      for (size_t k : todo) {
+        const std::string &key = keys[missing[k]];
+        srcs.push_back("#define QM_OCC " + std::to_string(occ) + "\n" + key.substr(key.find('\n') + 1));
+      }
+      std::vector<std::vector<char>> codes = compile_all(srcs);
+      std::vector<size_t> again;
+      for (size_t i = 0; i < todo.size(); i++) {
+        Compiled cp;
+        QM_HIP(hipModuleLoadData(&cp.mod, codes[i].data()));
+        QM_HIP(hipModuleGetFunction(&cp.fn, cp.mod, "qm_pass"));
+        int scratch = 0;
+        QM_HIP(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, cp.fn));
+        if (scratch > 0 && occ > 2) {
+          (void)hipModuleUnload(cp.mod);
+          again.push_back(todo[i]);
+          continue;
         }
-      });
-    for (auto &x : th) x.join();
+        int per_cu = 0;
+        QM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, cp.fn, 1 << plan.shape.L, jit_lds_bytes(precision, plan.shape.K)));
+        cp.resident = std::max(per_cu, 1);
+        done[todo[i]] = cp;
+      }
+      todo.swap(again);
+    }
     std::lock_guard<std::mutex> lk(c.mu);
     for (size_t k = 0; k < missing.size(); k++) {
-      if (codes[k].empty()) fail(QM_ERR_DEVICE, "pass compilation failed: " + errs[k]);
-      Compiled cp;
-      QM_HIP(hipModuleLoadData(&cp.mod, codes[k].data()));
-      QM_HIP(hipModuleGetFunction(&cp.fn, cp.mod, "qm_pass"));
-      c.map[keys[missing[k]]] = cp;
+      c.map[keys[missing[k]]] = done[k];
       c.compiled++;
     }
     c.compile_us += (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(
@@ -432,15 +509,18 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
                         .count();
   }
   std::lock_guard<std::mutex> lk(c.mu);
-  for (const std::string &k : keys) jp.fns.push_back((void *)c.map[k].fn);
+  for (const std::string &k : keys) {
+    jp.fns.push_back((void *)c.map[k].fn);
+    jp.resident.push_back(c.map[k].resident);
+  }
   return jp;
 }
 
 void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,
                 const double *d_coef, hipStream_t st) {
   const uint64_t tiles = 1ull << (s->qubits - shape.K);
-  const size_t lds = (size_t)amp_bytes(s->precision) << shape.K;
-  const uint64_t want = (uint64_t)s->ctx->compute_units * 2;
+  const size_t lds = jit_lds_bytes(s->precision, shape.K);
+  const uint64_t want = (uint64_t)s->ctx->compute_units * (uint64_t)jp.resident[pass];
   const unsigned blocks = (unsigned)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
   void *state = s->data;
   const double *cf = d_coef + jp.coef_offset[pass];
@@ -452,9 +532,9 @@ void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t 
 
 int jit_compile_check(const FusedPlan &plan, int precision) {
   const char *dump = std::getenv("QM_JIT_DUMP");  // debugging aid: directory for the sources
+  std::vector<std::string> srcs;
   int index = 0;
   for (const FusedPass &p : plan.passes) {
-    std::string err;
     std::vector<double> coef;
     const std::string src = pass_source(p, plan.shape, precision, coef);
     if (dump) {
@@ -470,9 +550,9 @@ int jit_compile_check(const FusedPlan &plan, int precision) {
       }
       index++;
     }
-    const std::vector<char> code = compile_one(std::string(kJitPrelude) + src, err);
-    if (code.empty()) fail(QM_ERR_DEVICE, "pass compilation failed: " + err);
+    srcs.push_back("#define QM_OCC 2\n" + std::string(kJitPrelude) + src);
   }
+  compile_all(srcs);
   return (int)plan.passes.size();
 }
 

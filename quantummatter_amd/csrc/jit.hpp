@@ -17,6 +17,7 @@ struct JitProgram {
   std::vector<void *> fns;             // hipFunction_t per pass
   std::vector<double> coef;            // all passes' coefficients
   std::vector<int64_t> coef_offset;    // first coefficient of each pass
+  std::vector<int> resident;           // workgroups per CU of each pass kernel
 };
 
 bool jit_supported(const FusedPlan &plan);  // no run-time resolved ops
