@@ -177,7 +177,7 @@ def main():
     n, L = args.qubits, args.layers
     circuit = workloads.random_layer_circuit(n, L, seed=1)
     ops = circuit.to_ops()
-    passes, transposes, gates = plan_fused(n, ops, prec)
+    gates = plan_fused(n, ops, prec)[2]
     amp_bytes = 16 if prec == C128 else 8
     with HipStateVectorBackend(HipBackendOptions(EnableProfiling=True)) as be:
         with be.CreateSession(n, prec) as s:
@@ -240,8 +240,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 random layer circuit, seed 1)",
         "config": {"workload": f"{n}q {args.precision} random layer circuit, {L} layers "
                                f"({gates} gates), {args.schedule} tile passes",
-                   "qubits": n, "layers": L, "gates": gates, "passes": passes,
-                   "passes_per_layer": passes / L, "lds_transposes": transposes,
+                   "qubits": n, "layers": L, "gates": gates, "passes": prof.Passes,
+                   "passes_per_layer": prof.Passes / L,
                    "device_ms_per_step": dev_ms, "schedule": args.schedule,
                    "pass_kernels_compiled": compiled.value, "compile_s": compile_s.value,
                    "warmup_s": t_warm},

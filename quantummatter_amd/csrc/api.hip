@@ -118,6 +118,8 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
   const double state_bytes = (double)s->bytes;
   FusedShape shape = c->shape[s->precision];
   int schedule = schedule_in;
+  if (schedule == QM_SCHED_COMPILED && s->qubits < shape.K && s->qubits >= 12)
+    shape = FusedShape{12, 8, 4, shape.contiguous};  // narrower states: 4096-amplitude tiles
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     if (jit_supported(plan)) {

@@ -76,8 +76,12 @@ struct Ctx {
   void *d_prog = nullptr;
   int64_t prog_cap = 0;
   bool disposed = false;
-  // Fused tile shape per precision (set by qm_ctx_set_fused_shape or the autotuner).
-  FusedShape shape[2] = {FusedShape{12, 8, 4, 3}, FusedShape{13, 8, 5, 4}};
+  // Tile shape per precision (qm_ctx_set_fused_shape).  The compiled schedule
+  // uses it as given; the interpreting kernel uses it when it is one of its
+  // instantiations and its own default otherwise.  Defaults: 8192-amplitude
+  // tiles, 256 threads x 32 registers (the compiled schedule's best on the
+  // 30-qubit layer workload).
+  FusedShape shape[2] = {FusedShape{13, 8, 5, 3}, FusedShape{13, 8, 5, 4}};
 };
 
 struct State {

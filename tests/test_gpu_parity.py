@@ -245,7 +245,7 @@ def test_uniform_superposition_sampling_is_exact():
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("n", [12, 13, 16])
+@pytest.mark.parametrize("n", [12, 13, 14, 16])
 def test_compiled_schedule_all_kinds(n):
     from quantummatter_amd.backend import SCHED_COMPILED
     c = random_all_kinds(n, 240, seed=100 + n)
