@@ -292,8 +292,8 @@ int qm_energy_batch(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_op 
 
 typedef struct qm_server_options {
   int32_t device_index;
-  int32_t full_lanes;          /* concurrent full-chip lanes (0: 2) */
-  int32_t batch_max_qubits;    /* widths batched (0: 16) */
+  int32_t full_lanes;          /* concurrent full-chip lanes (0: 3) */
+  int32_t batch_max_qubits;    /* widths batched (0: 14) */
   int32_t max_batch;           /* states per batched launch (0: 1024) */
   int64_t device_budget_bytes; /* admission budget (0: device memory less 4 GiB) */
 } qm_server_options;

@@ -65,7 +65,7 @@ class RequestResult:
 
 
 class HipRequestServer:
-    def __init__(self, device_index: int = 0, full_lanes: int = 2, batch_max_qubits: int = 16,
+    def __init__(self, device_index: int = 0, full_lanes: int = 3, batch_max_qubits: int = 14,
                  max_batch: int = 1024, device_budget_bytes: int = 0):
         o = QmServerOptions(device_index, full_lanes, batch_max_qubits, max_batch, device_budget_bytes)
         h = C.c_void_p()
