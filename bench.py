@@ -161,6 +161,25 @@ def batched_extras(be):
     return out
 
 
+def dense_extras(be, n=30, layers=10):
+    """The dense-block variant: gates fused into blocks of k <= 5 qubits, one
+    FP64 matrix-core pass per block, on the same generator (complex128)."""
+    from quantummatter_amd import workloads
+    from quantummatter_amd.backend import C128
+    c = workloads.random_layer_circuit(n, layers, seed=1)
+    out = {}
+    with be.CreateSession(n, C128) as s:
+        for k in (4, 5):
+            s.Reset()
+            p, flops = s.EvolveDense(c, k, profile=True)
+            t = p.DeviceEvolutionMs * 1e-3
+            out[f"k{k}"] = {"gate_layers_per_s": layers / t, "passes": p.Passes,
+                            "ms_per_pass": p.DeviceEvolutionMs / p.Passes,
+                            "algorithmic_GBps": p.AlgorithmicBytes / t / 1e9,
+                            "mfma_TFLOPs": flops / t / 1e12, "mfma_frac_fp64_peak": flops / t / 78.6e12}
+    return out
+
+
 def served_extras(be, count=48, full_lanes=3, batch_max_qubits=14):
     """The mixed request stream (widths log-uniform 10..28, depth 10..100 layers,
     amplitudes / 1024 shots / 64-term expectation), all arriving at once, while
@@ -279,6 +298,7 @@ def main():
         if not args.no_extras and not args.no_batched:
             extras["batched"] = batched_extras(be)
             extras["served"] = served_extras(be)
+            extras["dense_blocks_mfma"] = dense_extras(be, n)
     traffic = pmc_traffic(os.path.join(ROOT, "profiles",
                                        f"{args.schedule}_pass_{args.precision}_{n}q_pmc.txt"))
     line = {

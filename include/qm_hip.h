@@ -217,6 +217,19 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
  * summation order.  Returns the count accumulated on this state. */
 int qm_state_boundary_fallbacks(qm_state *state, int64_t *count);
 
+/* Gates fused into dense blocks of at most max_block_qubits (1..5) qubits,
+ * each block one pass of a 2^(k+1)-square real matrix product on the matrix
+ * cores (FP64 MFMA), complex128 only.  The reference has no fused form; this
+ * is the dense-block variant of IStateVectorBackend.Execute
+ * (src/QuantumMatter.Core/IStateVectorBackend.cs:13) on a resident state. */
+int qm_state_execute_dense(qm_state *state, const qm_op *ops, int64_t n_ops, int32_t max_block_qubits,
+                           qm_profile *profile);
+/* Matrix-core flops of the context's last dense execution. */
+int qm_dense_flops(qm_ctx *ctx, double *flops);
+/* Host reference of the dense-block schedule (qubits <= 26); *blocks = passes. */
+int qm_plan_dense_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t max_block_qubits,
+                             double *amplitudes, int32_t *blocks);
+
 /* Pass kernels compiled for QM_SCHED_COMPILED so far in this process, cache
  * hits, and the wall time spent compiling (no counterpart in the reference). */
 int qm_jit_stats(int64_t *compiled, int64_t *cache_hits, double *compile_seconds);

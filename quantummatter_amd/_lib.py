@@ -71,6 +71,9 @@ SIGNATURES = {
     "qm_energy_gradient_batch": [P, I32, I32, P, P, I64, P, I64, P, P, I64, P, P, P],
     "qm_energy_batch": [P, I32, I32, P, P, I64, P, I64, I64, P, P, I64, P, P],
     "qm_jit_stats": [P, P, P],
+    "qm_state_execute_dense": [P, P, I64, I32, P],
+    "qm_dense_flops": [P, P],
+    "qm_plan_dense_apply_host": [I32, P, I64, I32, P, P],
     "qm_jit_compile_check": [I32, I32, P, I64, I32, I32, P],
     "qm_server_create": [P, P],
     "qm_server_submit": [P, P, P],

@@ -241,6 +241,22 @@ class HipStateSession:
              len(u), ptr(cur))
         return int(cur[0])
 
+    def EvolveDense(self, circuit: QuantumCircuit, max_block_qubits: int = 4, profile: bool = False):
+        """Gates fused into dense blocks of at most max_block_qubits qubits, one
+        matrix-core pass per block (complex128).  Returns (profile, flops)."""
+        if circuit.QubitCount != self.QubitCount:
+            raise ArgumentError("Circuit and state sizes must match.")
+        ops = circuit.to_ops()
+        prof = QmProfile() if profile else None
+        call("qm_state_execute_dense", self._handle(), ptr(ops), len(ops), max_block_qubits,
+             None if prof is None else C.byref(prof))
+        flops = C.c_double()
+        call("qm_dense_flops", self._backend._handle(), C.byref(flops))
+        if prof is None:
+            return None, flops.value
+        return HipExecutionProfile(prof.device_ms, prof.state_bytes, prof.kernel_launches, prof.passes,
+                                   prof.algorithmic_bytes), flops.value
+
     def EvolveTrajectory(self, circuit: NoisyCircuit, uniforms: np.ndarray, cursor: int = 0) -> int:
         if circuit.QubitCount != self.QubitCount:
             raise ArgumentError("Circuit and state sizes must match.")
@@ -315,6 +331,14 @@ def plan_fused(qubits: int, ops: np.ndarray, precision: int = C128):
     p, t, g = C.c_int32(), C.c_int32(), C.c_int64()
     call("qm_plan_fused", qubits, precision, ptr(ops), len(ops), C.byref(p), C.byref(t), C.byref(g))
     return p.value, t.value, g.value
+
+
+def dense_apply_host(n: int, ops: np.ndarray, state: np.ndarray, max_block_qubits: int):
+    """Host reference of the dense-block schedule: (new state, blocks)."""
+    out = np.array(state, dtype=np.complex128, copy=True)
+    blocks = C.c_int32()
+    call("qm_plan_dense_apply_host", n, ptr(ops), len(ops), max_block_qubits, ptr(out), C.byref(blocks))
+    return out, blocks.value
 
 
 def plan_apply_host(qubits: int, ops: np.ndarray, state: np.ndarray, tile_bits: int, lane_bits: int,

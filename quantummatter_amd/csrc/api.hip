@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "dense.hpp"
 #include "jit.hpp"
 #include "qm_internal.hpp"
 
@@ -676,6 +677,72 @@ int qm_state_inner(qm_state *a, qm_state *b, double *re, double *im) {
     if (!re || !im) fail(QM_ERR_ARG, "outputs are null");
     std::lock_guard<std::recursive_mutex> g(x->ctx->lock);
     reduce_inner(x, y, re, im, x->ctx->stream);
+  });
+}
+
+int qm_state_execute_dense(qm_state *h, const qm_op *ops, int64_t n_ops, int32_t max_block_qubits,
+                           qm_profile *profile) {
+  return guarded([&] {
+    State *s = as_state(h);
+    Ctx *c = s->ctx;
+    if (profile && !c->profiling)
+      fail(QM_ERR_STATE, "EnableProfiling must be selected when creating the HIP backend.");
+    if (max_block_qubits < 1 || max_block_qubits > 5) fail(QM_ERR_RANGE, "block width must be 1..5");
+    if (s->precision != QM_C128) fail(QM_ERR_UNSUPPORTED, "dense blocks run on complex128 states");
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    std::vector<DevGate> gates = prepare_all(ops, n_ops, s->qubits);
+    std::vector<DenseBlock> blocks = plan_dense(gates, s->qubits, std::min<int>(max_block_qubits, s->qubits));
+    std::vector<double> host;
+    std::vector<int64_t> at;
+    for (const DenseBlock &b : blocks) {
+      at.push_back((int64_t)host.size());
+      host.insert(host.end(), b.a.begin(), b.a.end());
+    }
+    const int64_t bytes = (int64_t)(host.size() * sizeof(double));
+    double *d = (double *)ctx_program(c, bytes + 256);
+    if (bytes) QM_HIP(hipMemcpyAsync(d, host.data(), (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    if (profile) QM_HIP(hipEventRecord(c->ev_start, c->stream));
+    double flops = 0;
+    for (size_t i = 0; i < blocks.size(); i++) {
+      launch_dense(s, blocks[i], d + at[i], c->stream);
+      const double m = 2.0 * (double)(1 << blocks[i].k);
+      flops += 2.0 * m * m * (double)(s->length >> blocks[i].k);
+    }
+    if (profile) QM_HIP(hipEventRecord(c->ev_stop, c->stream));
+    QM_HIP(hipStreamSynchronize(c->stream));
+    if (profile) {
+      float ms = 0;
+      QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+      profile->device_ms = ms;
+      profile->state_bytes = s->bytes;
+      profile->kernel_launches = (int32_t)blocks.size();
+      profile->passes = (int32_t)blocks.size();
+      // Algorithmic bytes of the executed schedule; the matrix-core work is
+      // reported by qm_dense_flops for the same call.
+      profile->algorithmic_bytes = 2.0 * (double)s->bytes * (double)blocks.size();
+    }
+    c->last_dense_flops = flops;
+  });
+}
+
+int qm_dense_flops(qm_ctx *h, double *flops) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    if (!c || !flops) fail(QM_ERR_ARG, "null argument");
+    *flops = c->last_dense_flops;
+  });
+}
+
+int qm_plan_dense_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t max_block_qubits,
+                             double *amplitudes, int32_t *blocks_out) {
+  return guarded([&] {
+    if (qubits < 1 || qubits > 26) fail(QM_ERR_RANGE, "qubits must be between 1 and 26 for the host reference");
+    if (!amplitudes) fail(QM_ERR_ARG, "amplitudes is null");
+    if (max_block_qubits < 1 || max_block_qubits > 5) fail(QM_ERR_RANGE, "block width must be 1..5");
+    std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
+    std::vector<DenseBlock> blocks = plan_dense(gates, qubits, std::min<int>(max_block_qubits, qubits));
+    for (const DenseBlock &b : blocks) apply_dense_host(b, amplitudes, qubits);
+    if (blocks_out) *blocks_out = (int32_t)blocks.size();
   });
 }
 

@@ -604,6 +604,105 @@ BatchProgram plan_batch(const std::vector<BatchStep> &steps, int n, const FusedS
 }
 
 // ---------------------------------------------------------------------------
+// Dense blocks.
+// ---------------------------------------------------------------------------
+namespace {
+
+uint64_t gate_qubits(const DevGate &g) {
+  uint64_t q = ((uint64_t)1 << g.target) | g.ctrl_mask;
+  if (g.kind == G_SWAP) q |= (uint64_t)1 << g.swap_other;
+  return q;
+}
+
+// The gate with its qubits renamed to positions within the block.
+DevGate localise(const DevGate &g, uint64_t qmask) {
+  auto pos = [&](int q) { return popc(qmask & (((uint64_t)1 << q) - 1)); };
+  DevGate l = g;
+  l.target = pos(g.target);
+  if (g.kind == G_SWAP) l.swap_other = pos(g.swap_other);
+  l.ctrl_mask = 0;
+  for (int q = 0; q < 64; q++)
+    if (g.ctrl_mask & ((uint64_t)1 << q)) l.ctrl_mask |= (uint64_t)1 << pos(q);
+  return l;
+}
+
+}  // namespace
+
+std::vector<DenseBlock> plan_dense(const std::vector<DevGate> &gates, int n, int kmax) {
+  if (kmax < 1 || kmax > 5) throw std::invalid_argument("dense block width must be 1..5");
+  std::vector<DenseBlock> out;
+  uint64_t open = 0;
+  std::vector<DevGate> run;
+  auto flush = [&] {
+    if (run.empty()) return;
+    DenseBlock b;
+    b.qmask = open;
+    b.k = popc(open);
+    b.gates = (int)run.size();
+    const int D = 1 << b.k, M = 2 * D;
+    std::vector<double> col(2 * D);
+    b.a.assign((size_t)M * M, 0.0);
+    for (int c = 0; c < D; c++) {
+      std::fill(col.begin(), col.end(), 0.0);
+      col[2 * c] = 1.0;
+      for (const DevGate &g : run) apply_gate_host(localise(g, open), col.data(), b.k);
+      for (int r = 0; r < D; r++) {
+        const double re = col[2 * r], im = col[2 * r + 1];
+        b.a[(size_t)r * M + c] = re;
+        b.a[(size_t)r * M + D + c] = -im;
+        b.a[(size_t)(D + r) * M + c] = im;
+        b.a[(size_t)(D + r) * M + D + c] = re;
+      }
+    }
+    out.push_back(std::move(b));
+    run.clear();
+    open = 0;
+  };
+  for (const DevGate &g : gates) {
+    const uint64_t q = gate_qubits(g);
+    if (popc(open | q) > kmax) flush();
+    open |= q;
+    run.push_back(g);
+    if (popc(open) > kmax) flush();  // a gate wider than kmax is a block of its own
+  }
+  flush();
+  (void)n;
+  return out;
+}
+
+void apply_dense_host(const DenseBlock &b, double *s, int n) {
+  const int D = 1 << b.k, M = 2 * D;
+  std::vector<uint64_t> loc(D);
+  std::vector<int> qpos;
+  for (int q = 0; q < n; q++)
+    if (b.qmask & ((uint64_t)1 << q)) qpos.push_back(q);
+  for (int l = 0; l < D; l++) {
+    uint64_t off = 0;
+    for (int i = 0; i < b.k; i++)
+      if (l & (1 << i)) off |= (uint64_t)1 << qpos[i];
+    loc[l] = off;
+  }
+  std::vector<double> x(M), y(M);
+  const uint64_t len = (uint64_t)1 << n;
+  for (uint64_t base = 0; base < len; base++) {
+    if (base & b.qmask) continue;
+    for (int l = 0; l < D; l++) {
+      x[l] = s[2 * (base | loc[l])];
+      x[D + l] = s[2 * (base | loc[l]) + 1];
+    }
+    for (int r = 0; r < M; r++) {
+      double acc = 0;
+      for (int c = 0; c < M; c++) acc += b.a[(size_t)r * M + c] * x[c];
+      y[r] = acc;
+    }
+    for (int l = 0; l < D; l++) {
+      s[2 * (base | loc[l])] = y[l];
+      s[2 * (base | loc[l]) + 1] = y[D + l];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host references.
 // ---------------------------------------------------------------------------
 namespace {

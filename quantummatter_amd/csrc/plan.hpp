@@ -228,6 +228,26 @@ struct BatchProgram {
 
 BatchProgram plan_batch(const std::vector<BatchStep> &steps, int n, const FusedShape &shape);
 
+// ---------------------------------------------------------------------------
+// Dense blocks: consecutive gates whose qubits fit in k are multiplied into one
+// 2^k x 2^k complex matrix, applied to the state in one pass as a real
+// (2^(k+1) x 2^(k+1)) by (2^(k+1) x columns) product on the matrix cores.
+// ---------------------------------------------------------------------------
+struct DenseBlock {
+  uint64_t qmask = 0;     // block qubits (local bit b = b-th lowest qubit of the mask)
+  int k = 0;              // popcount(qmask), 1..5
+  int gates = 0;          // gates multiplied into the block
+  std::vector<double> a;  // real form [[Re U, -Im U], [Im U, Re U]], row-major, 2^(k+1) square
+};
+
+// Greedy blocking in gate order: a gate joins the open block while the union of
+// qubits stays within kmax (kmax 1..5); gates acting on more qubits than kmax
+// form a block of their own.
+std::vector<DenseBlock> plan_dense(const std::vector<DevGate> &gates, int n, int kmax);
+
+// Host reference of the dense pass (the kernel's index mapping, FP64 sums).
+void apply_dense_host(const DenseBlock &b, double *state, int n);
+
 // Host reference of the kernels, used by the planner's CPU tests: applies a
 // plan / one gate to an interleaved complex128 state with the kernels'
 // arithmetic and indexing.

@@ -76,6 +76,7 @@ struct Ctx {
   void *d_prog = nullptr;
   int64_t prog_cap = 0;
   bool disposed = false;
+  double last_dense_flops = 0;  // matrix-core flops of the last dense execution
   // Tile shape per precision (qm_ctx_set_fused_shape).  The compiled schedule
   // uses it as given; the interpreting kernel uses it when it is one of its
   // instantiations and its own default otherwise.  Defaults: 8192-amplitude

@@ -282,3 +282,28 @@ def test_compiled_schedule_c64_infidelity():
         s.Evolve(c, SCHED_COMPILED)
         got = s.Snapshot()
     assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-5
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("n", [10, 13])
+def test_dense_blocks_on_matrix_cores(n, k):
+    c = random_all_kinds(n, 160, seed=200 + 7 * n + k)
+    ops = c.to_ops()
+    rng = np.random.default_rng(k)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    want = oracle.execute(n, ops, psi)
+    with session(n, state=psi) as s:
+        s.EvolveDense(c, k)
+        got = s.Snapshot()
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_dense_blocks_layer_circuit_16q():
+    n = 16
+    c = workloads.random_layer_circuit(n, 10, seed=8)
+    want = oracle.execute(n, c.to_ops())
+    with session(n) as s:
+        s.EvolveDense(c, 5)
+        got = s.Snapshot()
+    assert np.max(np.abs(got - want)) <= 1e-12
