@@ -269,6 +269,20 @@ int qm_trajectories_run(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm
                         int64_t *outcomes, int64_t *consumed, int64_t *fallbacks,
                         qm_profile *profile);
 
+/* NoisyStateVectorSimulator.EstimateProbabilities (src/QuantumMatter.Application/
+ * Noise/NoisyStateVectorSimulator.cs:64-106) over `trajectories` (>= 2)
+ * trajectories, trajectory t drawing from pools[t * pool_stride ...] as in
+ * qm_trajectories_run (no terminal measurement).  means[j] is the Welford mean
+ * of the normalised probability of basis index j over the trajectories in
+ * order, std_errors[j] = sqrt(max(0, M2_j) / (M (M - 1))); both are 2^qubits
+ * doubles.  consumed[t] (optional) = draws taken from pool t.  Errors:
+ * QM_ERR_RANGE for fewer than 2 trajectories, QM_ERR_POOL, QM_ERR_MASS. */
+int qm_trajectories_estimate(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_step *steps,
+                             int64_t n_steps, const qm_op *ops, int64_t n_ops,
+                             int64_t trajectories, const double *pools, int64_t pool_stride,
+                             double *means, double *std_errors, int64_t *consumed,
+                             int64_t *fallbacks, qm_profile *profile);
+
 /* VariationalEnergyEvaluator.EvaluateWithGradient (src/QuantumMatter.Application/
  * Variational/VariationalEnergyEvaluator.cs:34-86): the energy of the bound
  * circuit and the parameter-shift gradient, all 1 + 2R circuits (R = rotation

@@ -87,6 +87,40 @@ class HipTrajectoryRunner:
              None if prof is None else C.byref(prof))
         return TrajectoryResult(samples, outcomes, consumed, fb.value, _profile(prof))
 
+    def EstimateProbabilities(self, circuit: NoisyCircuit, pools: np.ndarray,
+                              profile: bool = False) -> ProbabilityEstimate:
+        """EstimateProbabilities (NoisyStateVectorSimulator.cs:64-106): trajectory t
+        draws from pools[t]; means and standard errors of the Born probabilities."""
+        if circuit is None:
+            raise ArgumentError("circuit is null.")
+        pools = np.ascontiguousarray(pools, dtype=np.float64)
+        if pools.ndim != 2:
+            raise ArgumentError("pools must be [trajectories, stride].")
+        T, stride = pools.shape
+        steps, ops = circuit.packed()
+        n = circuit.QubitCount
+        means = np.zeros(1 << n, dtype=np.float64)
+        se = np.zeros(1 << n, dtype=np.float64)
+        consumed = np.zeros(T, dtype=np.int64)
+        fb = C.c_int64()
+        prof = QmProfile() if profile else None
+        call("qm_trajectories_estimate", self._backend._handle(), n, self._precision,
+             ptr(steps), len(steps), ptr(ops), len(ops), T, ptr(pools), stride, ptr(means), ptr(se),
+             ptr(consumed), C.byref(fb), None if prof is None else C.byref(prof))
+        return ProbabilityEstimate(n, T, means, se, consumed, fb.value, _profile(prof))
+
+
+@dataclass
+class ProbabilityEstimate:
+    """NoisyProbabilityResult (Noise/NoisyProbabilityResult.cs) plus the draw accounting."""
+    QubitCount: int
+    Trajectories: int
+    Means: np.ndarray            # [2^n] Welford means of the normalised probabilities
+    StandardErrors: np.ndarray   # [2^n] sqrt(M2 / (M (M - 1)))
+    consumed: np.ndarray         # [trajectories] draws taken from each pool
+    boundary_fallbacks: int
+    profile: Optional[HipExecutionProfile]
+
 
 class HipBatchedEnergyEvaluator:
     """Variational energies and parameter-shift gradients, batched on the device."""

@@ -56,6 +56,45 @@ struct Arena {
   }
 };
 
+// Welford update of NoisyStateVectorSimulator.EstimateProbabilities
+// (NoisyStateVectorSimulator.cs:72-90) over `count` consecutive trajectories
+// whose states sit at states + t * len, trajectory `first` being the first of
+// them.  Thread j owns basis index j and walks the trajectories in order, so
+// every index sees the reference's sequence of updates; the probability is
+// re*re + im*im without contraction (StateVector.cs:56-60).
+template <typename T>
+__global__ __launch_bounds__(256) void k_welford(const T *__restrict__ states, int64_t len,
+                                                 int64_t count, const double *__restrict__ mass,
+                                                 int64_t first, double *__restrict__ mean,
+                                                 double *__restrict__ m2) {
+#pragma clang fp contract(off)
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += stride) {
+    double mu = mean[j], v = m2[j];
+    for (int64_t t = 0; t < count; t++) {
+      const T a = states[t * len + j];
+      const double re = a.x, im = a.y;
+      const double p = (re * re + im * im) / mass[t];
+      const double delta = p - mu;
+      mu += delta / (double)(first + t + 1);
+      v += delta * (p - mu);
+    }
+    mean[j] = mu;
+    m2[j] = v;
+  }
+}
+
+// Standard errors in place of M2: sqrt(max(0, M2) / (M (M - 1)))
+// (NoisyStateVectorSimulator.cs:92-101); a NaN M2 stays NaN, as Math.Max keeps it.
+__global__ __launch_bounds__(256) void k_std_error(double *__restrict__ m2, int64_t len,
+                                                   double divisor) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += stride) {
+    const double v = m2[j];
+    m2[j] = sqrt((v < 0 ? 0.0 : v) / divisor);
+  }
+}
+
 }  // namespace
 
 void run_batch_launch(Ctx *c, BatchLaunch &L) {
@@ -156,7 +195,7 @@ void run_batch_launch(Ctx *c, BatchLaunch &L) {
   down(L.outcomes.data(), o_outcomes, L.outcomes.size() * 8);
   down(L.consumed.data(), o_consumed, L.consumed.size() * 8);
   down(&fb, o_fb, 8);
-  if (L.keep && L.on_states) L.on_states(ar.at<void>(o_states), st);
+  if (L.keep && L.on_states) L.on_states(ar.at<void>(o_states), ar.at<double>(o_mass), st);
   QM_HIP(hipStreamSynchronize(st));
   if (L.timed) {
     float ms = 0;
@@ -306,6 +345,141 @@ double energy_of(const BatchOut &o, int64_t item, const double *coeffs, int64_t 
   return e;
 }
 
+// NoisyCircuit.Execute (NoisyCircuit.cs:48-65) as batch steps: segments to
+// gates, channels with p != 0 to branches (CpuNoiseChannelBackend.cs:10-83).
+std::vector<BatchStep> noisy_steps(int qubits, const qm_step *steps, int64_t n_steps,
+                                   const qm_op *ops, int64_t n_ops) {
+  if (n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "steps are null");
+  std::vector<BatchStep> out;
+  for (int64_t k = 0; k < n_steps; k++) {
+    const qm_step &s = steps[k];
+    if (!s.is_channel) {
+      if (s.op_begin < 0 || s.op_count < 0 || s.op_begin + s.op_count > n_ops)
+        fail(QM_ERR_ARG, "bad segment");
+      for (const DevGate &g : prepare_all(ops + s.op_begin, s.op_count, qubits)) {
+        BatchStep bs;
+        bs.gate = g;
+        out.push_back(bs);
+      }
+      continue;
+    }
+    const double p = s.probability;
+    if (p == 0) continue;
+    if (s.target < 0 || s.target >= qubits) fail(QM_ERR_RANGE, "target");
+    if (!std::isfinite(p) || p < 0 || p > 1)
+      fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
+    if (s.noise_kind < 0 || s.noise_kind > 3) fail(QM_ERR_ARG, "unknown noise kind");
+    BatchStep bs;
+    if (s.noise_kind == 3) {
+      bs.damp_target = s.target;
+      bs.gamma = p;
+    } else {
+      DevGate &g = bs.gate;
+      std::memset(&g, 0, sizeof(g));
+      g.kind = G_NOISE;
+      g.target = s.target;
+      g.swap_other = -1;
+      g.aux0 = (s.noise_kind != 2 && p == 1) ? -1 : 0;  // flips at p == 1 draw nothing
+      g.aux1 = s.noise_kind;
+      g.c[0] = p;
+    }
+    out.push_back(bs);
+  }
+  return out;
+}
+
+// EstimateProbabilities over batched trajectories: chunks of trajectories
+// are evolved in one launch each (states kept), then one Welford pass per
+// chunk folds their normalised probabilities into the running mean and M2 in
+// trajectory order.
+struct EstimateOut {
+  int64_t fallbacks = 0;
+  int passes = 0;
+  int64_t chunks = 0;
+  double device_ms = 0;
+};
+
+void run_estimate(Ctx *c, int n, int precision, const std::vector<BatchStep> &steps,
+                  int64_t trajectories, const double *pools, int64_t stride, double *means,
+                  double *std_errors, int64_t *consumed, bool timed, EstimateOut &out) {
+  const FusedShape shape = batch_shape(precision, n);
+  if (shape.K == 0) fail(QM_ERR_UNSUPPORTED, "Batched execution needs at least 8 qubits.");
+  const BatchProgram prog = plan_batch(steps, n, shape);
+  const int64_t len = (int64_t)1 << n;
+  const int64_t ab = amp_bytes(precision);
+  const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
+  const int64_t room = limit - c->resident_bytes - 16 * len - (256ll << 20);
+  const int64_t chunk = std::min<int64_t>({trajectories, 2048, room / (len * ab)});
+  if (chunk < 1) fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
+  double *acc = nullptr;  // [mean | m2], 2 * len doubles
+  if (hipMalloc(&acc, (size_t)(16 * len)) != hipSuccess) {
+    (void)hipGetLastError();
+    fail(QM_ERR_BUDGET, "HIP batch exceeds the free device memory.");
+  }
+  struct Free {
+    double *p;
+    ~Free() { (void)hipFree(p); }
+  } guard{acc};
+  hipStream_t st = c->stream;
+  QM_HIP(hipMemsetAsync(acc, 0, (size_t)(16 * len), st));
+  const unsigned grid = (unsigned)std::min<int64_t>((len + 255) / 256, (int64_t)c->compute_units * 8);
+  if (timed) QM_HIP(hipEventRecord(c->ev_start, st));
+  for (int64_t first = 0; first < trajectories; first += chunk) {
+    const int64_t count = std::min(chunk, trajectories - first);
+    BatchLaunch L;
+    L.precision = precision;
+    L.shape = shape;
+    L.prog = prog;
+    L.keep = true;
+    L.keep_len = count * len;
+    L.items.resize((size_t)count);
+    for (int64_t i = 0; i < count; i++) {
+      BatchItem &it = L.items[(size_t)i];
+      it = BatchItem{};
+      it.state_off = i * len;
+      it.len = (uint64_t)len;
+      it.tiles = (uint64_t)len >> shape.K;
+      it.seg_count = (int32_t)prog.segs.size();
+      it.want_mass = 1;
+    }
+    L.pools = pools ? pools + first * stride : nullptr;
+    L.stride = stride;
+    L.on_states = [&](const void *states, const double *mass, hipStream_t s) {
+      if (precision == QM_C128)
+        k_welford<double2><<<grid, 256, 0, s>>>(static_cast<const double2 *>(states), len, count,
+                                                mass, first, acc, acc + len);
+      else
+        k_welford<float2><<<grid, 256, 0, s>>>(static_cast<const float2 *>(states), len, count,
+                                               mass, first, acc, acc + len);
+      QM_HIP(hipGetLastError());
+    };
+    run_batch_launch(c, L);
+    for (int64_t i = 0; i < count; i++) {
+      const int code = L.err[(size_t)i];
+      if (code == QM_ERR_POOL) fail(code, "The uniform pool is exhausted.");
+      if (code == QM_ERR_MASS || (!code && !valid_mass(L.mass[(size_t)i])))
+        fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+      if (code) fail(code, "batched state failed");
+    }
+    if (consumed) std::memcpy(consumed + first, L.consumed.data(), (size_t)count * 8);
+    out.fallbacks += L.fallbacks;
+    out.chunks++;
+  }
+  const double divisor = (double)trajectories * (double)(trajectories - 1);
+  k_std_error<<<grid, 256, 0, st>>>(acc + len, len, divisor);
+  QM_HIP(hipGetLastError());
+  if (timed) QM_HIP(hipEventRecord(c->ev_stop, st));
+  QM_HIP(hipMemcpyAsync(means, acc, (size_t)(8 * len), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipMemcpyAsync(std_errors, acc + len, (size_t)(8 * len), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  if (timed) {
+    float ms = 0;
+    QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+    out.device_ms = ms;
+  }
+  out.passes = (int)prog.passes.size();
+}
+
 // Template ops of a ParameterizedCircuit (ParameterizedCircuit.cs:68-96) as
 // batch steps: parameter rotations read their coefficients from the angle table.
 struct Template {
@@ -399,7 +573,6 @@ int qm_trajectories_run(qm_ctx *h, int32_t qubits, int32_t precision, const qm_s
     Ctx *c = batch_ctx(h, profile);
     check_width(qubits, precision);
     if (trajectories < 1) fail(QM_ERR_RANGE, "trajectories must be positive.");
-    if (n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "steps are null");
     if (pool_stride < 0 || (pool_stride > 0 && !pools)) fail(QM_ERR_ARG, "pools are null");
     if (shots < 0 || (shots > 0 && (!shot_uniforms || !samples))) fail(QM_ERR_ARG, "shot buffers are null");
     if (measure_all && !outcomes) fail(QM_ERR_ARG, "outcomes are null");
@@ -413,46 +586,44 @@ int qm_trajectories_run(qm_ctx *h, int32_t qubits, int32_t precision, const qm_s
     job.shot_u = shot_uniforms;
     job.shots = shots;
     job.measure = measure_all != 0;
-    // NoisyCircuit.Execute (NoisyCircuit.cs:48-65): segments to gates, channels with
-    // p != 0 to branches (CpuNoiseChannelBackend.cs:10-83).
-    for (int64_t k = 0; k < n_steps; k++) {
-      const qm_step &s = steps[k];
-      if (!s.is_channel) {
-        if (s.op_begin < 0 || s.op_count < 0 || s.op_begin + s.op_count > n_ops)
-          fail(QM_ERR_ARG, "bad segment");
-        for (const DevGate &g : prepare_all(ops + s.op_begin, s.op_count, qubits)) {
-          BatchStep bs;
-          bs.gate = g;
-          job.steps.push_back(bs);
-        }
-        continue;
-      }
-      const double p = s.probability;
-      if (p == 0) continue;
-      if (s.target < 0 || s.target >= qubits) fail(QM_ERR_RANGE, "target");
-      if (!std::isfinite(p) || p < 0 || p > 1)
-        fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
-      if (s.noise_kind < 0 || s.noise_kind > 3) fail(QM_ERR_ARG, "unknown noise kind");
-      BatchStep bs;
-      if (s.noise_kind == 3) {
-        bs.damp_target = s.target;
-        bs.gamma = p;
-      } else {
-        DevGate &g = bs.gate;
-        std::memset(&g, 0, sizeof(g));
-        g.kind = G_NOISE;
-        g.target = s.target;
-        g.swap_other = -1;
-        g.aux0 = (s.noise_kind != 2 && p == 1) ? -1 : 0;  // flips at p == 1 draw nothing
-        g.aux1 = s.noise_kind;
-        g.c[0] = p;
-      }
-      job.steps.push_back(bs);
-    }
+    job.steps = noisy_steps(qubits, steps, n_steps, ops, n_ops);
     BatchOut out;
     run_batch(c, job, samples, outcomes, consumed, profile != nullptr, out);
     if (fallbacks) *fallbacks = out.fallbacks;
     fill_profile(profile, out);
+  });
+}
+
+int qm_trajectories_estimate(qm_ctx *h, int32_t qubits, int32_t precision, const qm_step *steps,
+                             int64_t n_steps, const qm_op *ops, int64_t n_ops,
+                             int64_t trajectories, const double *pools, int64_t pool_stride,
+                             double *means, double *std_errors, int64_t *consumed,
+                             int64_t *fallbacks, qm_profile *profile) {
+  return guarded([&] {
+    Ctx *c = batch_ctx(h, profile);
+    check_width(qubits, precision);
+    if (trajectories < 2) fail(QM_ERR_RANGE, "trajectories must be at least 2.");
+    if (pool_stride < 0 || (pool_stride > 0 && !pools)) fail(QM_ERR_ARG, "pools are null");
+    if (!means || !std_errors) fail(QM_ERR_ARG, "outputs are null");
+    std::lock_guard<std::recursive_mutex> lk(c->lock);
+    const std::vector<BatchStep> bsteps = noisy_steps(qubits, steps, n_steps, ops, n_ops);
+    EstimateOut out;
+    run_estimate(c, qubits, precision, bsteps, trajectories, pool_stride > 0 ? pools : nullptr,
+                 pool_stride, means, std_errors, consumed, profile != nullptr, out);
+    if (fallbacks) *fallbacks = out.fallbacks;
+    if (profile) {
+      const int64_t state_bytes = amp_bytes(precision) << qubits;
+      profile->device_ms = out.device_ms;
+      profile->state_bytes = state_bytes;
+      profile->kernel_launches = (int32_t)(2 * out.chunks + 1);
+      profile->passes = out.passes;
+      // Per trajectory: every pass reads and writes the state, the mass and the
+      // Welford pass read it; per chunk the running mean and M2 are read and
+      // written; the standard-error pass reads and writes M2.
+      const double len = std::ldexp(1.0, qubits);
+      profile->algorithmic_bytes = (double)trajectories * (double)state_bytes * (2.0 * out.passes + 2.0) +
+                                   (double)out.chunks * 32.0 * len + 16.0 * len;
+    }
   });
 }
 

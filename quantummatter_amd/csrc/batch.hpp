@@ -66,7 +66,8 @@ TermGroups group_terms(const qm_pauli *terms, int64_t n_terms);
 // outputs (per item status, numerators in group order, masses, samples,
 // outcomes, draw counts).  With `keep`, item states live at
 // items[i].state_off in one buffer of keep_len amplitudes and `on_states` is
-// called with it (on the context's stream) before the arena is released.
+// called with it and the device array of item masses (on the context's
+// stream, after the batch kernel) before the arena is released.
 struct BatchLaunch {
   int precision = QM_C128;
   FusedShape shape{};
@@ -84,7 +85,7 @@ struct BatchLaunch {
   int32_t slots = 0;
   const double *shot_u = nullptr;  // [n_shots]
   bool timed = false;
-  std::function<void(const void *states, hipStream_t stream)> on_states;
+  std::function<void(const void *states, const double *mass, hipStream_t stream)> on_states;
   std::vector<int32_t> err;
   std::vector<double> numer, mass;
   std::vector<int64_t> samples, outcomes, consumed;

@@ -215,7 +215,7 @@ void run_batched(Ctx *c, const std::vector<Req *> &group, const FusedShape &shap
   }
   L.shot_u = shot_u.data();
   const int64_t ab = amp_bytes(L.precision);
-  L.on_states = [&](const void *states, hipStream_t) {
+  L.on_states = [&](const void *states, const double *, hipStream_t) {
     for (size_t i = 0; i < group.size(); i++)
       if (group[i]->kind == QM_REQ_AMPLITUDES)
         download(c, group[i]->out, static_cast<const char *>(states) + L.items[i].state_off * ab,
