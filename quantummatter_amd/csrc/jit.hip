@@ -8,6 +8,12 @@
 // coefficients), so re-running a circuit structure with new angles reuses them.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <dlfcn.h>
+#include <spawn.h>
+#include <sys/wait.h>
+#include <unistd.h>
+
+#include <cerrno>
 
 #include <algorithm>
 #include <atomic>
@@ -24,6 +30,7 @@
 #include <vector>
 
 #include "jit.hpp"
+#include "rtc.hpp"
 #include "qm_internal.hpp"
 
 namespace qm {
@@ -386,45 +393,98 @@ JitCache &cache() {
   return c;
 }
 
-// hipRTC to a code object; empty result with the log in `err` on failure.
-std::vector<char> compile_one(const std::string &src, std::string &err) {
-  hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "qm_pass.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
-    err = "hiprtcCreateProgram failed";
-    return {};
-  }
-  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-Xclang", "-target-feature",
-                        "-Xclang", "-packed-fp32-ops"};
-  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
-  std::vector<char> code;
-  if (rc != HIPRTC_SUCCESS) {
-    size_t n = 0;
-    hiprtcGetProgramLogSize(prog, &n);
-    std::string log(n, '\0');
-    if (n) hiprtcGetProgramLog(prog, &log[0]);
-    err = std::string(hiprtcGetErrorString(rc)) + ": " + log.substr(0, 4000);
-  } else {
-    size_t n = 0;
-    hiprtcGetCodeSize(prog, &n);
-    code.resize(n);
-    hiprtcGetCode(prog, code.data());
-  }
-  hiprtcDestroyProgram(&prog);
-  return code;
+// Directory of libqmhip.so, where the build puts qm_jitc beside it.
+std::string helper_path() {
+  Dl_info info{};
+  if (!dladdr((const void *)&helper_path, &info) || !info.dli_fname) return {};
+  std::string lib = info.dli_fname;
+  const size_t slash = lib.rfind('/');
+  return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/qm_jitc";
 }
 
-// Compiles sources on 16 worker threads (hipRTC compiles scale with threads).
+std::vector<char> read_file(const std::string &path) {
+  std::vector<char> out;
+  FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f) return out;
+  char buf[65536];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) out.insert(out.end(), buf, buf + n);
+  std::fclose(f);
+  return out;
+}
+
+// Compiles the sources in up to `workers` qm_jitc child processes (the
+// sources and code objects pass through a private temporary directory).
+// Returns false, with nothing compiled, when the helper cannot be started.
+bool compile_spawned(const std::vector<std::string> &srcs, size_t workers,
+                     std::vector<std::vector<char>> &codes, std::vector<std::string> &errs) {
+  const std::string helper = helper_path();
+  if (helper.empty() || access(helper.c_str(), X_OK) != 0) return false;
+  const char *tmp = std::getenv("TMPDIR");
+  std::string templ = std::string(tmp && *tmp ? tmp : "/tmp") + "/qmjit.XXXXXX";
+  std::vector<char> dir_buf(templ.begin(), templ.end());
+  dir_buf.push_back('\0');
+  if (!mkdtemp(dir_buf.data())) return false;
+  const std::string dir = dir_buf.data();
+  auto file = [&](size_t k, const char *ext) { return dir + "/k" + std::to_string(k) + ext; };
+  bool ok = true;
+  for (size_t k = 0; k < srcs.size() && ok; k++) {
+    FILE *f = std::fopen(file(k, ".hip").c_str(), "wb");
+    ok = f && std::fwrite(srcs[k].data(), 1, srcs[k].size(), f) == srcs[k].size();
+    if (f) std::fclose(f);
+  }
+  std::vector<pid_t> pids;
+  for (size_t w = 0; w < workers && ok; w++) {
+    std::vector<std::string> args{helper, dir};
+    for (size_t k = w; k < srcs.size(); k += workers) args.push_back(std::to_string(k));
+    std::vector<char *> argv;
+    for (std::string &a : args) argv.push_back(&a[0]);
+    argv.push_back(nullptr);
+    pid_t pid = 0;
+    if (posix_spawn(&pid, helper.c_str(), nullptr, nullptr, argv.data(), environ) != 0) {
+      ok = false;
+      break;
+    }
+    pids.push_back(pid);
+  }
+  for (pid_t pid : pids) {
+    int st = 0;
+    while (waitpid(pid, &st, 0) < 0 && errno == EINTR) {
+    }
+  }
+  const bool spawned = ok;
+  for (size_t k = 0; k < srcs.size(); k++) {
+    if (spawned) {
+      codes[k] = read_file(file(k, ".co"));
+      if (codes[k].empty()) {
+        const std::vector<char> e = read_file(file(k, ".err"));
+        errs[k] = e.empty() ? "the compile helper produced no code object" : std::string(e.begin(), e.end());
+      }
+    }
+    for (const char *ext : {".hip", ".co", ".err", ".tmp"}) std::remove(file(k, ext).c_str());
+  }
+  rmdir(dir.c_str());
+  return spawned;
+}
+
+// Compiles every source: in helper processes when there are several (hipRTC
+// compiles one program at a time per process), else on threads in-process.
+// QM_JIT_PROCS caps the helper count (0: compile in-process).
 std::vector<std::vector<char>> compile_all(const std::vector<std::string> &srcs) {
   std::vector<std::vector<char>> codes(srcs.size());
   std::vector<std::string> errs(srcs.size());
-  std::atomic<size_t> next{0};
-  const size_t workers = std::min<size_t>(srcs.size(), 16);
-  std::vector<std::thread> th;
-  for (size_t w = 0; w < workers; w++)
-    th.emplace_back([&] {
-      for (size_t k; (k = next++) < srcs.size();) codes[k] = compile_one(srcs[k], errs[k]);
-    });
-  for (auto &x : th) x.join();
+  size_t procs = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+  if (const char *e = std::getenv("QM_JIT_PROCS")) procs = (size_t)std::max(0, std::atoi(e));
+  const size_t workers = std::min(srcs.size(), procs);
+  if (workers < 2 || !compile_spawned(srcs, workers, codes, errs)) {
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> th;
+    for (size_t w = 0; w < std::max<size_t>(std::min<size_t>(srcs.size(), 16), 1); w++)
+      th.emplace_back([&] {
+        for (size_t k; (k = next++) < srcs.size();) codes[k] = rtc_compile(srcs[k], errs[k]);
+      });
+    for (auto &x : th) x.join();
+  }
   for (size_t k = 0; k < srcs.size(); k++)
     if (codes[k].empty()) fail(QM_ERR_DEVICE, "pass compilation failed: " + errs[k]);
   return codes;
