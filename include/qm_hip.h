@@ -137,6 +137,28 @@ int qm_ctx_resident_bytes(qm_ctx *ctx, int64_t *bytes);
 int qm_ctx_set_fused_shape(qm_ctx *ctx, int32_t precision, int32_t tile_bits, int32_t lane_bits);
 int qm_ctx_get_fused_shape(qm_ctx *ctx, int32_t precision, int32_t *tile_bits, int32_t *lane_bits);
 
+/* No reference counterpart: the autotuner.  Times every tile shape the chosen
+ * schedule (QM_SCHED_FUSED: each interpreter instantiation and its code mode;
+ * QM_SCHED_COMPILED: generated kernels with 2^11..2^14-amplitude tiles) on a
+ * scratch state of `qubits` qubits running a prefix of the given circuit (at
+ * most 32 x qubits gates), and keeps the fastest for that schedule, precision
+ * and width in the context: later executions of that width use it, until
+ * qm_ctx_set_fused_shape replaces it.  entries[0..min(capacity, *count)) get
+ * one row per candidate (wall milliseconds of the prefix, best of two runs),
+ * *best the winning row.  Results do not depend on the shape beyond rounding
+ * (fused and compiled schedules stay within 1e-12 of the per-gate one). */
+typedef struct qm_tune_entry {
+  int32_t tile_bits;
+  int32_t lane_bits;
+  int32_t variant;  /* interpreter mode / code-generation variant, -1 = default */
+  int32_t passes;
+  double ms;
+  double ms_per_gate;
+} qm_tune_entry;
+int qm_autotune(qm_ctx *ctx, int32_t qubits, int32_t precision, int32_t schedule, const qm_op *ops,
+                int64_t n_ops, qm_tune_entry *entries, int32_t capacity, int32_t *count,
+                int32_t *best);
+
 /* ---- drop-in whole-circuit execution ------------------------------------ */
 /* IStateVectorBackend.Execute (src/QuantumMatter.Core/IStateVectorBackend.cs:13)
  * as implemented by CudaStateVectorBackend.ExecuteCore (:103-188): validates

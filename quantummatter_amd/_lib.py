@@ -23,6 +23,11 @@ class QmDeviceInfo(C.Structure):
                 ("name", C.c_char * 128), ("arch", C.c_char * 64), ("driver", C.c_char * 64)]
 
 
+class QmTuneEntry(C.Structure):
+    _fields_ = [("tile_bits", C.c_int32), ("lane_bits", C.c_int32), ("variant", C.c_int32),
+                ("passes", C.c_int32), ("ms", C.c_double), ("ms_per_gate", C.c_double)]
+
+
 class QmCtxOptions(C.Structure):
     _fields_ = [("device_index", C.c_int32), ("enable_profiling", C.c_int32),
                 ("device_budget_bytes", C.c_int64), ("staging_budget_bytes", C.c_int64)]
@@ -43,6 +48,7 @@ SIGNATURES = {
     "qm_ctx_destroy": [P],
     "qm_ctx_resident_bytes": [P, P],
     "qm_ctx_set_fused_shape": [P, I32, I32, I32],
+    "qm_autotune": [P, I32, I32, I32, P, I64, P, I32, P, P],
     "qm_ctx_get_fused_shape": [P, I32, P, P],
     "qm_execute": [P, I32, P, I64, P, I64, P, P],
     "qm_state_create": [P, I32, I32, P],

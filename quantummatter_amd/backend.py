@@ -137,6 +137,21 @@ class HipStateVectorBackend:
         call("qm_ctx_get_fused_shape", self._handle(), precision, C.byref(k), C.byref(l))
         return k.value, l.value
 
+    def Autotune(self, circuit: QuantumCircuit, precision: int = C128, schedule: int = SCHED_FUSED):
+        """Times every tile shape of `schedule` on a prefix of `circuit` and keeps the
+        fastest for this width and precision (qm_autotune).  Returns (best, rows), each
+        row a dict of tile_bits, lane_bits, variant, passes, ms, ms_per_gate."""
+        from ._lib import QmTuneEntry
+        ops = circuit.to_ops()
+        cap = 64
+        rows = (QmTuneEntry * cap)()
+        count, best = C.c_int32(), C.c_int32()
+        with self._lock:
+            call("qm_autotune", self._handle(), circuit.QubitCount, precision, schedule, ptr(ops), len(ops),
+                 rows, cap, C.byref(count), C.byref(best))
+        out = [{f: getattr(rows[i], f) for f, _ in QmTuneEntry._fields_} for i in range(min(count.value, cap))]
+        return out[best.value], out
+
     def ResidentBytes(self) -> int:
         b = C.c_int64()
         call("qm_ctx_resident_bytes", self._handle(), C.byref(b))

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -113,14 +114,20 @@ std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
 
 // Runs prepared gates on a resident state with the chosen schedule.
 ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in,
-                    const volatile int32_t *cancel) {
+                    const volatile int32_t *cancel, const FusedShape *force) {
   Ctx *c = s->ctx;
   ExecStats st;
   const double state_bytes = (double)s->bytes;
   FusedShape shape = c->shape[s->precision];
   int schedule = schedule_in;
-  if (schedule == QM_SCHED_COMPILED && s->qubits < shape.K && s->qubits >= 12)
+  const int tuned_slot = schedule == QM_SCHED_FUSED ? 0 : schedule == QM_SCHED_COMPILED ? 1 : -1;
+  if (force) {
+    shape = *force;
+  } else if (tuned_slot >= 0 && c->tuned[tuned_slot][s->precision][s->qubits].K) {
+    shape = c->tuned[tuned_slot][s->precision][s->qubits];
+  } else if (schedule == QM_SCHED_COMPILED && s->qubits < shape.K && s->qubits >= 12) {
     shape = FusedShape{12, 8, 4, shape.contiguous};  // narrower states: 4096-amplitude tiles
+  }
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     if (jit_supported(plan)) {
@@ -788,6 +795,8 @@ int qm_ctx_set_fused_shape(qm_ctx *h, int32_t precision, int32_t tile_bits, int3
       fail(QM_ERR_UNSUPPORTED, "fused tile shape is not supported");
     std::lock_guard<std::recursive_mutex> g(c->lock);
     c->shape[precision] = FusedShape{tile_bits, lane_bits, R, precision == QM_C128 ? 3 : 4};
+    for (auto &per_schedule : c->tuned)  // an explicit shape replaces tuned ones
+      for (FusedShape &t : per_schedule[precision]) t = FusedShape{};
   });
 }
 
@@ -798,6 +807,76 @@ int qm_ctx_get_fused_shape(qm_ctx *h, int32_t precision, int32_t *tile_bits, int
     if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
     if (tile_bits) *tile_bits = c->shape[precision].K;
     if (lane_bits) *lane_bits = c->shape[precision].L;
+  });
+}
+
+int qm_autotune(qm_ctx *h, int32_t qubits, int32_t precision, int32_t schedule, const qm_op *ops,
+                int64_t n_ops, qm_tune_entry *entries, int32_t capacity, int32_t *count,
+                int32_t *best) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    use_device(c);
+    if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
+    if (schedule != QM_SCHED_FUSED && schedule != QM_SCHED_COMPILED)
+      fail(QM_ERR_ARG, "only the fused and compiled schedules have tile shapes");
+    if (qubits < 1 || qubits > 34) fail(QM_ERR_RANGE, "qubits must be between 1 and 34.");
+    if (capacity < 0 || (capacity > 0 && !entries)) fail(QM_ERR_ARG, "entries are null");
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
+    // A prefix of a few layers is representative and keeps tuning short.
+    if (gates.size() > (size_t)(32 * qubits)) gates.resize((size_t)(32 * qubits));
+    if (gates.size() < 2) fail(QM_ERR_ARG, "the circuit has fewer than two gates to tune on");
+    const int contiguous = precision == QM_C128 ? 3 : 4;
+    std::vector<FusedShape> cands;
+    if (schedule == QM_SCHED_FUSED) {
+      for (const FusedShape &f : fused_shapes(precision))
+        if (f.K <= qubits) cands.push_back(f);
+    } else {
+      // Generated kernels: tiles of 2^11..2^13 (c128) or 2^12..2^14 (c64)
+      // amplitudes with 128..512 threads, in the default code variant.
+      const int kl[][2] = {{11, 7}, {12, 7}, {12, 8}, {12, 9}, {13, 8}, {13, 9}, {14, 9}, {14, 10}};
+      for (const auto &p : kl) {
+        const int K = p[0], L = p[1], R = K - L;
+        if (K > qubits || R < 2 || R > 5 || (precision == QM_C128 && K > 13) || K < 12 - (precision == QM_C128))
+          continue;
+        cands.push_back(FusedShape{K, L, R, contiguous});
+      }
+    }
+    if (cands.empty()) fail(QM_ERR_UNSUPPORTED, "no tile shape fits this width");
+    State *s = state_new(c, qubits, precision);
+    struct Free {
+      State *s;
+      ~Free() { state_free(s); }
+    } guard{s};
+    std::vector<qm_tune_entry> out;
+    int32_t winner = 0;
+    for (size_t i = 0; i < cands.size(); i++) {
+      const FusedShape &f = cands[i];
+      launch_set_basis(s, 0, c->stream);
+      ExecStats st = run_gates(s, gates, schedule, nullptr, &f);  // compiles, warms caches
+      double ms = 0;
+      for (int rep = 0; rep < 2; rep++) {
+        launch_set_basis(s, 0, c->stream);
+        QM_HIP(hipStreamSynchronize(c->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        run_gates(s, gates, schedule, nullptr, &f);
+        const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        ms = rep == 0 ? t : std::min(ms, t);
+      }
+      qm_tune_entry e{};
+      e.tile_bits = f.K;
+      e.lane_bits = f.L;
+      e.variant = f.variant;
+      e.passes = st.passes;
+      e.ms = ms;
+      e.ms_per_gate = ms / (double)gates.size();
+      out.push_back(e);
+      if (ms < out[(size_t)winner].ms) winner = (int32_t)i;
+    }
+    c->tuned[schedule == QM_SCHED_FUSED ? 0 : 1][precision][qubits] = cands[(size_t)winner];
+    for (int32_t i = 0; i < std::min<int32_t>(capacity, (int32_t)out.size()); i++) entries[i] = out[(size_t)i];
+    if (count) *count = (int32_t)out.size();
+    if (best) *best = winner;
   });
 }
 

@@ -174,6 +174,17 @@ bool fused_shape_supported(int precision, int L, int R) {
   return false;
 }
 
+std::vector<FusedShape> fused_shapes(int precision) {
+  std::vector<FusedShape> out;
+  const int contiguous = precision == QM_C128 ? 3 : 4;
+#define QM_LIST(T, l, r, m)                                                   \
+  if ((precision == QM_C128) == std::is_same<T, C128>::value)                \
+    out.push_back(FusedShape{l + r, l, r, contiguous, m});
+  QM_FUSED_SHAPES(QM_LIST)
+#undef QM_LIST
+  return out;
+}
+
 FusedShape default_fused_shape(int precision) {
   // complex128: 4096-amplitude tiles (64 KiB LDS), 256 threads x 16 registers.
   // complex64: 8192-amplitude tiles (64 KiB LDS), 256 threads x 32 registers.
@@ -191,7 +202,7 @@ static int mode_override() {
 
 void launch_fused_pass(State *s, const FusedShape &shape, const void *d_pass, const void *d_ops,
                        hipStream_t st) {
-  const int want = mode_override();
+  const int want = shape.variant >= 0 ? shape.variant : mode_override();
   bool seen = false;
   // First pass: the requested mode; second: the shape's default (first listed).
   for (int round = 0; round < 2; round++) {

@@ -172,14 +172,16 @@ int jit_variant() {
   return v;
 }
 
-size_t jit_lds_bytes(int precision, int K) {
-  const bool split = (jit_variant() & 1) != 0;
-  return (split ? (size_t)amp_bytes(precision) / 2 : (size_t)amp_bytes(precision)) << K;
+int variant_of(const FusedShape &sh) { return sh.variant >= 0 ? sh.variant : jit_variant(); }
+
+size_t jit_lds_bytes(int precision, const FusedShape &sh) {
+  const bool split = (variant_of(sh) & 1) != 0;
+  return (split ? (size_t)amp_bytes(precision) / 2 : (size_t)amp_bytes(precision)) << sh.K;
 }
 
 std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
                         std::vector<double> &coef) {
-  const int variant = jit_variant();
+  const int variant = variant_of(sh);
   const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
@@ -553,7 +555,7 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
         }
         int per_cu = 0;
         QM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, cp.fn, 1 << plan.shape.L, jit_lds_bytes(precision, plan.shape.K)));
+            &per_cu, cp.fn, 1 << plan.shape.L, jit_lds_bytes(precision, plan.shape)));
         cp.resident = std::max(per_cu, 1);
         done[todo[i]] = cp;
       }
@@ -579,7 +581,7 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
 void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,
                 const double *d_coef, hipStream_t st) {
   const uint64_t tiles = 1ull << (s->qubits - shape.K);
-  const size_t lds = jit_lds_bytes(s->precision, shape.K);
+  const size_t lds = jit_lds_bytes(s->precision, shape);
   const uint64_t want = (uint64_t)s->ctx->compute_units * (uint64_t)jp.resident[pass];
   const unsigned blocks = (unsigned)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
   void *state = s->data;

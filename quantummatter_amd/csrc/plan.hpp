@@ -169,6 +169,7 @@ struct FusedShape {
   int L;           // lane bits (threads = 2^L)
   int R;           // register bits
   int contiguous;  // lowest qubits always in the tile
+  int variant = -1;  // code variant: interpreter mode or generator variant (-1: default)
 };
 
 struct FusedPass {

@@ -83,6 +83,9 @@ struct Ctx {
   // tiles, 256 threads x 32 registers (the compiled schedule's best on the
   // 30-qubit layer workload).
   FusedShape shape[2] = {FusedShape{13, 8, 5, 3}, FusedShape{13, 8, 5, 4}};
+  // Shapes chosen by qm_autotune, [fused 0 / compiled 1][precision][qubits];
+  // K == 0: not tuned.  They take precedence over `shape` for their width.
+  FusedShape tuned[2][2][35] = {};
 };
 
 struct State {
@@ -111,8 +114,9 @@ struct ExecStats {
   double bytes = 0;
 };
 // Runs prepared gates on a state with the chosen schedule (synchronous).
+// `force` (optional) replaces the context's tile shape for this call.
 ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule,
-                    const volatile int32_t *cancel);
+                    const volatile int32_t *cancel, const FusedShape *force = nullptr);
 
 // Scratch management on the context's stream.
 double *ctx_partials(Ctx *c, int64_t n);
@@ -125,6 +129,7 @@ void launch_gate(State *s, const DevGate &g, hipStream_t st);
 void launch_fused_pass(State *s, const FusedShape &shape, const void *d_pass, const void *d_ops,
                        hipStream_t st);
 FusedShape default_fused_shape(int precision);
+std::vector<FusedShape> fused_shapes(int precision);  // every interpreter instantiation
 bool fused_shape_supported(int precision, int L, int R);
 void launch_set_basis(State *s, int64_t index, hipStream_t st);  // |index>
 void launch_scale_pairs(State *s, int target, bool jump, double scale, double jump_scale,
