@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form extras")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream extras")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense-block extras")
     ap.add_argument("--schedule", choices=["compiled", "fused"], default="compiled",
                     help="compiled: fused passes as straight-line kernels (compiled during warmup, "
                          "cached by circuit structure); fused: the interpreting tile-pass kernel")
@@ -121,7 +123,7 @@ def batched_extras(be):
     import numpy as np
     from quantummatter_amd import workloads
     from quantummatter_amd.backend import C64, C128
-    from quantummatter_amd.batched import HipBatchedEnergyEvaluator, HipTrajectoryRunner
+    from quantummatter_amd.batched import HipBatchedEnergyEvaluator, HipTrajectoryRunner, HipVqeProgram
     out = {}
     n, L, T, shots = 16, 30, 4096, 1024
     nc = workloads.noisy_layer_circuit(n, L, seed=3)
@@ -154,10 +156,19 @@ def batched_extras(be):
             p = ev.last_profile
             circuits = 1 + 2 * circ.ParameterCount
             gbps = p.AlgorithmicBytes / (p.DeviceEvolutionMs * 1e-3) / 1e9
+            # The same gradient replayed from the captured graph of a VQE program
+            # (arena kept, angles uploaded from pinned memory), mean of 5 calls.
+            with HipVqeProgram(be, circ, ham, prec) as prog:
+                prog.EvaluateWithGradient(circ, ham, theta)
+                t1 = time.perf_counter()
+                for _ in range(5):
+                    prog.EvaluateWithGradient(circ, ham, theta)
+                replay = (time.perf_counter() - t1) / 5
             out[f"vqe_gradient_{q}q_{name}"] = {
                 "circuits_per_s": circuits / wall, "circuits": circuits, "wall_s": wall,
                 "device_ms": p.DeviceEvolutionMs, "algorithmic_GBps": gbps,
-                "hbm_frac": gbps / HBM_PEAK_GBS}
+                "hbm_frac": gbps / HBM_PEAK_GBS,
+                "graph_replay_circuits_per_s": circuits / replay, "graph_replay_wall_s": replay}
     return out
 
 
@@ -297,7 +308,9 @@ def main():
                 }
         if not args.no_extras and not args.no_batched:
             extras["batched"] = batched_extras(be)
+        if not args.no_extras and not args.no_served:
             extras["served"] = served_extras(be)
+        if not args.no_extras and not args.no_dense:
             extras["dense_blocks_mfma"] = dense_extras(be, n)
     traffic = pmc_traffic(os.path.join(ROOT, "profiles",
                                        f"{args.schedule}_pass_{args.precision}_{n}q_pmc.txt"))

@@ -325,6 +325,28 @@ int qm_energy_batch(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_op 
                     int64_t n_params, int64_t n_sets, const qm_pauli *terms, const double *coeffs,
                     int64_t n_terms, double *energies, qm_profile *profile);
 
+/* A variational program for repeated evaluation (a VQE loop): the template
+ * and Hamiltonian of qm_energy_gradient_batch / qm_energy_batch, planned once.
+ * Its first gradient call (and the first energy call for each n_sets) builds
+ * a batch whose device arena stays allocated and captures it as a hipGraph
+ * (angle upload from pinned memory, the batch kernel, result downloads to
+ * pinned memory); later calls write the new angles and replay the graph.
+ * Results equal those of the one-shot entry points for the same parameters.
+ * The program holds its arenas until qm_vqe_program_destroy; calls on one
+ * program are serialised by its context.  Terms and coefficients are copied. */
+typedef struct qm_vqe_program qm_vqe_program;
+int qm_vqe_program_create(qm_ctx *ctx, int32_t qubits, int32_t precision, const qm_op *ops,
+                          const int32_t *param_index, int64_t n_ops, int64_t n_params,
+                          const qm_pauli *terms, const double *coeffs, int64_t n_terms,
+                          qm_vqe_program **out);
+/* VariationalEnergyEvaluator.EvaluateWithGradient (VariationalEnergyEvaluator.cs:34-86). */
+int qm_vqe_program_gradient(qm_vqe_program *program, const double *parameters, double *energy,
+                            double *gradient, qm_profile *profile);
+/* VariationalEnergyEvaluator.Evaluate (VariationalEnergyEvaluator.cs:23-32) for n_sets vectors. */
+int qm_vqe_program_energies(qm_vqe_program *program, const double *parameter_sets, int64_t n_sets,
+                            double *energies, qm_profile *profile);
+int qm_vqe_program_destroy(qm_vqe_program *program);
+
 /* ---------------------------------------------------------------------------
  * Request server (no counterpart in the reference, which runs one circuit per
  * call): requests of 1..34 qubits are admitted in arrival order when their

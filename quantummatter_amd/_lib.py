@@ -77,6 +77,10 @@ SIGNATURES = {
     "qm_trajectories_estimate": [P, I32, I32, P, I64, P, I64, I64, P, I64, P, P, P, P, P],
     "qm_energy_gradient_batch": [P, I32, I32, P, P, I64, P, I64, P, P, I64, P, P, P],
     "qm_energy_batch": [P, I32, I32, P, P, I64, P, I64, I64, P, P, I64, P, P],
+    "qm_vqe_program_create": [P, I32, I32, P, P, I64, I64, P, P, I64, P],
+    "qm_vqe_program_gradient": [P, P, P, P, P],
+    "qm_vqe_program_energies": [P, P, I64, P, P],
+    "qm_vqe_program_destroy": [P],
     "qm_jit_stats": [P, P, P],
     "qm_state_execute_dense": [P, P, I64, I32, P],
     "qm_dense_flops": [P, P],

@@ -7,7 +7,8 @@ NoisyStateVectorSimulator.cs:36-62) in one launch of ``qm_trajectories_run``;
 EvaluateWithGradient (src/QuantumMatter.Application/Variational/
 VariationalEnergyEvaluator.cs:23-86) with all 1 + 2R circuits in one launch
 of ``qm_energy_gradient_batch`` (or many parameter vectors with
-``qm_energy_batch``).  Each state is owned by one workgroup; results match the
+``qm_energy_batch``); ``HipVqeProgram`` replays the same evaluations from
+captured hipGraphs for a VQE loop.  Each state is owned by one workgroup; results match the
 per-state session path (and the reference's CPU path) under the same
 host-supplied uniforms.
 """
@@ -22,7 +23,7 @@ import numpy as np
 from ._lib import QmProfile, call, ptr
 from .backend import C128, HipExecutionProfile, HipStateVectorBackend
 from .circuit import NoisyCircuit, ParameterizedCircuit, PauliHamiltonian
-from .errors import ArgumentError
+from .errors import ArgumentError, ObjectDisposedError
 from .variational import VariationalEnergyResult
 
 
@@ -176,3 +177,74 @@ class HipBatchedEnergyEvaluator:
              ptr(coeffs), len(coeffs), ptr(out), None if prof is None else C.byref(prof))
         self.last_profile = _profile(prof)
         return out
+
+
+class HipVqeProgram:
+    """One template and Hamiltonian evaluated many times (a VQE loop) through
+    graphed batches (qm_vqe_program_*): the first gradient call, and the first
+    energy call for each number of parameter sets, capture a hipGraph that
+    later calls replay with new angles.  The evaluator methods take the
+    circuit and Hamiltonian for interface parity and require the program's own."""
+
+    def __init__(self, backend: HipStateVectorBackend, circuit: ParameterizedCircuit,
+                 hamiltonian: PauliHamiltonian, precision: int = C128):
+        if backend is None:
+            raise ArgumentError("backend is null.")
+        HipBatchedEnergyEvaluator._validate(circuit, hamiltonian)
+        self._backend = backend
+        self._circuit = circuit
+        self._hamiltonian = hamiltonian
+        self._n_params = circuit.ParameterCount
+        ops, pidx = circuit.packed()
+        paulis, coeffs = hamiltonian.packed()
+        self._coeffs = coeffs
+        h = C.c_void_p()
+        call("qm_vqe_program_create", backend._handle(), circuit.QubitCount, precision, ptr(ops), ptr(pidx),
+             len(ops), circuit.ParameterCount, ptr(paulis), ptr(coeffs), len(coeffs), C.byref(h))
+        self._h = h
+        self.last_profile: Optional[HipExecutionProfile] = None
+
+    def _check(self, circuit, hamiltonian):
+        if self._h is None:
+            raise ObjectDisposedError("HipVqeProgram")
+        if circuit is not self._circuit or hamiltonian is not self._hamiltonian:
+            raise ArgumentError("The program was built for another circuit or Hamiltonian.")
+
+    def EvaluateWithGradient(self, circuit, hamiltonian, parameters, profile: bool = False) -> VariationalEnergyResult:
+        self._check(circuit, hamiltonian)
+        prm = np.ascontiguousarray(parameters, dtype=np.float64)
+        if len(prm) != self._n_params:
+            raise ArgumentError("Binding size must match ParameterCount.")
+        energy = C.c_double()
+        grad = np.zeros(len(prm), dtype=np.float64)
+        prof = QmProfile() if profile else None
+        call("qm_vqe_program_gradient", self._h, ptr(prm), C.byref(energy), ptr(grad),
+             None if prof is None else C.byref(prof))
+        self.last_profile = _profile(prof)
+        return VariationalEnergyResult(energy.value, [float(g) for g in grad])
+
+    def EvaluateMany(self, circuit, hamiltonian, parameter_sets, profile: bool = False) -> np.ndarray:
+        self._check(circuit, hamiltonian)
+        sets = np.ascontiguousarray(parameter_sets, dtype=np.float64)
+        if sets.ndim != 2 or sets.shape[1] != self._n_params:
+            raise ArgumentError("Binding size must match ParameterCount.")
+        out = np.zeros(sets.shape[0], dtype=np.float64)
+        prof = QmProfile() if profile else None
+        call("qm_vqe_program_energies", self._h, ptr(sets), sets.shape[0], ptr(out),
+             None if prof is None else C.byref(prof))
+        self.last_profile = _profile(prof)
+        return out
+
+    def Evaluate(self, circuit, hamiltonian, parameters) -> float:
+        return float(self.EvaluateMany(circuit, hamiltonian, [parameters])[0])
+
+    def Dispose(self) -> None:
+        if self._h is not None:
+            call("qm_vqe_program_destroy", self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.Dispose()

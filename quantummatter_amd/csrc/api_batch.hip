@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <numeric>
 #include <vector>
 
@@ -95,85 +97,111 @@ __global__ __launch_bounds__(256) void k_std_error(double *__restrict__ m2, int6
   }
 }
 
+// Clears the fallback counter inside a captured graph (a kernel node rather
+// than a memset node).
+__global__ void k_zero_counter(unsigned long long *p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
 }  // namespace
+
+// Device arena of one batched launch: every input the kernel reads and every
+// output it writes, at fixed offsets, in one allocation.
+struct BatchArena {
+  Arena ar;
+  int64_t slots = 0;
+  size_t o_states = 0, o_items = 0, o_passes = 0, o_ops = 0, o_segs = 0, o_pools = 0, o_angles = 0,
+         o_groups = 0, o_numer = 0, o_mass = 0, o_shotu = 0, o_samples = 0, o_outcomes = 0,
+         o_consumed = 0, o_err = 0, o_fb = 0;
+
+  // Sizes the arena for L (workgroup slots within the budget unless L.keep),
+  // allocates it and uploads the inputs on the context's stream.
+  void setup(Ctx *c, const BatchLaunch &L) {
+    const int64_t n_items = (int64_t)L.items.size();
+    const int64_t ab = amp_bytes(L.precision);
+    if (!L.keep) {
+      // Workgroup slots: as many states as can be resident at once, within the budget.
+      slots = std::min<int64_t>(n_items, (int64_t)c->compute_units * 2);
+      const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
+      const int64_t room = limit - c->resident_bytes - (64ll << 20);
+      slots = std::min<int64_t>(slots, room / std::max<int64_t>(L.slot_len * ab, 1));
+      if (slots < 1) fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
+    }
+    const int64_t state_amps = L.keep ? L.keep_len : slots * L.slot_len;
+    o_states = ar.add((size_t)(state_amps * ab));
+    o_items = ar.add(L.items.size() * sizeof(BatchItem));
+    o_passes = ar.add(L.prog.passes.size() * sizeof(DevPass));
+    o_ops = ar.add(std::max<size_t>(L.prog.ops.size(), 1) * sizeof(TileOp));
+    o_segs = ar.add(L.prog.segs.size() * sizeof(BatchSeg));
+    o_pools = ar.add(L.pools ? (size_t)(n_items * L.stride) * 8 : 0);
+    o_angles = ar.add(L.angles.size() * 8);
+    o_groups = ar.add(std::max<size_t>(L.groups.size(), 1) * sizeof(PauliGroup));
+    o_numer = ar.add((size_t)std::max<int64_t>(L.n_numer, 1) * 8);
+    o_mass = ar.add((size_t)n_items * 8);
+    o_shotu = ar.add((size_t)std::max<int64_t>(L.n_shots, 1) * 8);
+    o_samples = ar.add((size_t)std::max<int64_t>(L.n_shots, 1) * 8);
+    o_outcomes = ar.add((size_t)n_items * 8);
+    o_consumed = ar.add((size_t)n_items * 8);
+    o_err = ar.add((size_t)n_items * 4);
+    o_fb = ar.add(8);
+    if ((c->device_budget && c->resident_bytes + (int64_t)ar.size > c->device_budget) ||
+        c->resident_bytes + (int64_t)ar.size > c->device_memory)
+      fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
+    hipError_t e = hipMalloc(&ar.base, ar.size);
+    if (e != hipSuccess) {
+      ar.base = nullptr;
+      (void)hipGetLastError();
+      fail(QM_ERR_BUDGET, "HIP batch exceeds the free device memory.");
+    }
+    hipStream_t st = c->stream;
+    auto up = [&](size_t off, const void *src, size_t bytes) {
+      if (bytes) QM_HIP(hipMemcpyAsync(ar.at<char>(off), src, bytes, hipMemcpyHostToDevice, st));
+    };
+    up(o_items, L.items.data(), L.items.size() * sizeof(BatchItem));
+    up(o_passes, L.prog.passes.data(), L.prog.passes.size() * sizeof(DevPass));
+    up(o_ops, L.prog.ops.data(), L.prog.ops.size() * sizeof(TileOp));
+    up(o_segs, L.prog.segs.data(), L.prog.segs.size() * sizeof(BatchSeg));
+    if (L.pools) up(o_pools, L.pools, (size_t)(n_items * L.stride) * 8);
+    up(o_angles, L.angles.data(), L.angles.size() * 8);
+    up(o_groups, L.groups.data(), L.groups.size() * sizeof(PauliGroup));
+    if (L.n_shots) up(o_shotu, L.shot_u, (size_t)L.n_shots * 8);
+  }
+
+  BatchArgs args(const BatchLaunch &L) const {
+    BatchArgs a{};
+    a.states = ar.at<void>(o_states);
+    a.slot_len = L.slot_len;
+    a.keep = L.keep ? 1 : 0;
+    a.slots = L.slots;
+    a.items = ar.at<BatchItem>(o_items);
+    a.n_items = (int64_t)L.items.size();
+    a.passes = ar.at<DevPass>(o_passes);
+    a.ops = ar.at<TileOp>(o_ops);
+    a.segs = ar.at<BatchSeg>(o_segs);
+    a.pools = L.pools ? ar.at<double>(o_pools) : nullptr;
+    a.stride = L.stride;
+    a.angles = L.angles.empty() ? nullptr : ar.at<double>(o_angles);
+    a.groups = ar.at<PauliGroup>(o_groups);
+    a.numer = ar.at<double>(o_numer);
+    a.mass = ar.at<double>(o_mass);
+    a.shot_u = ar.at<double>(o_shotu);
+    a.samples = ar.at<int64_t>(o_samples);
+    a.outcomes = ar.at<int64_t>(o_outcomes);
+    a.consumed = ar.at<int64_t>(o_consumed);
+    a.err = ar.at<int32_t>(o_err);
+    a.fallbacks = ar.at<unsigned long long>(o_fb);
+    return a;
+  }
+};
 
 void run_batch_launch(Ctx *c, BatchLaunch &L) {
   const int64_t n_items = (int64_t)L.items.size();
-  const int64_t ab = amp_bytes(L.precision);
-  int64_t slots = 0;
-  if (!L.keep) {
-    // Workgroup slots: as many states as can be resident at once, within the budget.
-    slots = std::min<int64_t>(n_items, (int64_t)c->compute_units * 2);
-    const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
-    const int64_t room = limit - c->resident_bytes - (64ll << 20);
-    slots = std::min<int64_t>(slots, room / std::max<int64_t>(L.slot_len * ab, 1));
-    if (slots < 1) fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
-  }
-  const int64_t state_amps = L.keep ? L.keep_len : slots * L.slot_len;
-  Arena ar;
-  const size_t o_states = ar.add((size_t)(state_amps * ab));
-  const size_t o_items = ar.add(L.items.size() * sizeof(BatchItem));
-  const size_t o_passes = ar.add(L.prog.passes.size() * sizeof(DevPass));
-  const size_t o_ops = ar.add(std::max<size_t>(L.prog.ops.size(), 1) * sizeof(TileOp));
-  const size_t o_segs = ar.add(L.prog.segs.size() * sizeof(BatchSeg));
-  const size_t o_pools = ar.add(L.pools ? (size_t)(n_items * L.stride) * 8 : 0);
-  const size_t o_angles = ar.add(L.angles.size() * 8);
-  const size_t o_groups = ar.add(std::max<size_t>(L.groups.size(), 1) * sizeof(PauliGroup));
-  const size_t o_numer = ar.add((size_t)std::max<int64_t>(L.n_numer, 1) * 8);
-  const size_t o_mass = ar.add((size_t)n_items * 8);
-  const size_t o_shotu = ar.add((size_t)std::max<int64_t>(L.n_shots, 1) * 8);
-  const size_t o_samples = ar.add((size_t)std::max<int64_t>(L.n_shots, 1) * 8);
-  const size_t o_outcomes = ar.add((size_t)n_items * 8);
-  const size_t o_consumed = ar.add((size_t)n_items * 8);
-  const size_t o_err = ar.add((size_t)n_items * 4);
-  const size_t o_fb = ar.add(8);
-  if ((c->device_budget && c->resident_bytes + (int64_t)ar.size > c->device_budget) ||
-      c->resident_bytes + (int64_t)ar.size > c->device_memory)
-    fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
-  hipError_t e = hipMalloc(&ar.base, ar.size);
-  if (e != hipSuccess) {
-    ar.base = nullptr;
-    (void)hipGetLastError();
-    fail(QM_ERR_BUDGET, "HIP batch exceeds the free device memory.");
-  }
+  BatchArena A;
+  A.setup(c, L);
   hipStream_t st = c->stream;
-  auto up = [&](size_t off, const void *src, size_t bytes) {
-    if (bytes) QM_HIP(hipMemcpyAsync(ar.at<char>(off), src, bytes, hipMemcpyHostToDevice, st));
-  };
-  up(o_items, L.items.data(), L.items.size() * sizeof(BatchItem));
-  up(o_passes, L.prog.passes.data(), L.prog.passes.size() * sizeof(DevPass));
-  up(o_ops, L.prog.ops.data(), L.prog.ops.size() * sizeof(TileOp));
-  up(o_segs, L.prog.segs.data(), L.prog.segs.size() * sizeof(BatchSeg));
-  if (L.pools) up(o_pools, L.pools, (size_t)(n_items * L.stride) * 8);
-  up(o_angles, L.angles.data(), L.angles.size() * 8);
-  up(o_groups, L.groups.data(), L.groups.size() * sizeof(PauliGroup));
-  if (L.n_shots) up(o_shotu, L.shot_u, (size_t)L.n_shots * 8);
-  QM_HIP(hipMemsetAsync(ar.at<char>(o_fb), 0, 8, st));
-
-  BatchArgs a{};
-  a.states = ar.at<void>(o_states);
-  a.slot_len = L.slot_len;
-  a.keep = L.keep ? 1 : 0;
-  a.slots = L.slots;
-  a.items = ar.at<BatchItem>(o_items);
-  a.n_items = n_items;
-  a.passes = ar.at<DevPass>(o_passes);
-  a.ops = ar.at<TileOp>(o_ops);
-  a.segs = ar.at<BatchSeg>(o_segs);
-  a.pools = L.pools ? ar.at<double>(o_pools) : nullptr;
-  a.stride = L.stride;
-  a.angles = L.angles.empty() ? nullptr : ar.at<double>(o_angles);
-  a.groups = ar.at<PauliGroup>(o_groups);
-  a.numer = ar.at<double>(o_numer);
-  a.mass = ar.at<double>(o_mass);
-  a.shot_u = ar.at<double>(o_shotu);
-  a.samples = ar.at<int64_t>(o_samples);
-  a.outcomes = ar.at<int64_t>(o_outcomes);
-  a.consumed = ar.at<int64_t>(o_consumed);
-  a.err = ar.at<int32_t>(o_err);
-  a.fallbacks = ar.at<unsigned long long>(o_fb);
-
-  int64_t grid = L.keep ? 0 : slots;
+  QM_HIP(hipMemsetAsync(A.ar.at<char>(A.o_fb), 0, 8, st));
+  const BatchArgs a = A.args(L);
+  int64_t grid = L.keep ? 0 : A.slots;
   if (L.timed) QM_HIP(hipEventRecord(c->ev_start, st));
   launch_batch(c, L.precision, L.shape, a, &grid);
   if (L.timed) QM_HIP(hipEventRecord(c->ev_stop, st));
@@ -186,16 +214,16 @@ void run_batch_launch(Ctx *c, BatchLaunch &L) {
   L.consumed.assign((size_t)n_items, 0);
   unsigned long long fb = 0;
   auto down = [&](void *dst, size_t off, size_t bytes) {
-    if (bytes) QM_HIP(hipMemcpyAsync(dst, ar.at<char>(off), bytes, hipMemcpyDeviceToHost, st));
+    if (bytes) QM_HIP(hipMemcpyAsync(dst, A.ar.at<char>(off), bytes, hipMemcpyDeviceToHost, st));
   };
-  down(L.err.data(), o_err, L.err.size() * 4);
-  down(L.numer.data(), o_numer, L.numer.size() * 8);
-  down(L.mass.data(), o_mass, L.mass.size() * 8);
-  down(L.samples.data(), o_samples, L.samples.size() * 8);
-  down(L.outcomes.data(), o_outcomes, L.outcomes.size() * 8);
-  down(L.consumed.data(), o_consumed, L.consumed.size() * 8);
-  down(&fb, o_fb, 8);
-  if (L.keep && L.on_states) L.on_states(ar.at<void>(o_states), ar.at<double>(o_mass), st);
+  down(L.err.data(), A.o_err, L.err.size() * 4);
+  down(L.numer.data(), A.o_numer, L.numer.size() * 8);
+  down(L.mass.data(), A.o_mass, L.mass.size() * 8);
+  down(L.samples.data(), A.o_samples, L.samples.size() * 8);
+  down(L.outcomes.data(), A.o_outcomes, L.outcomes.size() * 8);
+  down(L.consumed.data(), A.o_consumed, L.consumed.size() * 8);
+  down(&fb, A.o_fb, 8);
+  if (L.keep && L.on_states) L.on_states(A.ar.at<void>(A.o_states), A.ar.at<double>(A.o_mass), st);
   QM_HIP(hipStreamSynchronize(st));
   if (L.timed) {
     float ms = 0;
@@ -243,17 +271,15 @@ struct BatchOut {
   int64_t state_bytes = 0;
 };
 
-void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes, int64_t *consumed,
-               bool timed, BatchOut &out) {
+// The launch description of a job: plan, items, groups and inputs.
+TermGroups build_launch(const BatchJob &job, BatchLaunch &L) {
   const int n = job.qubits;
-  BatchLaunch L;
   L.precision = job.precision;
   L.shape = batch_shape(job.precision, n);
   if (L.shape.K == 0) fail(QM_ERR_UNSUPPORTED, "Batched execution needs at least 8 qubits.");
   L.prog = plan_batch(job.steps, n, L.shape);
   TermGroups g = group_terms(job.terms, job.n_terms);
   L.groups = g.groups;
-  const int64_t state_bytes = amp_bytes(job.precision) << n;
   L.slot_len = (int64_t)1 << n;
   L.items.resize((size_t)job.items);
   for (int64_t i = 0; i < job.items; i++) {
@@ -278,32 +304,45 @@ void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes,
   L.angles = job.angles;
   L.slots = job.slots;
   L.shot_u = job.shot_u;
-  L.timed = timed;
-  run_batch_launch(c, L);
+  return g;
+}
+
+// Per-item statuses to errors; masses and numerators (original term order) to `out`.
+void collect_out(const BatchJob &job, const TermGroups &g, const int32_t *err, const double *numer,
+                 const double *mass, int passes, BatchOut &out) {
   for (int64_t i = 0; i < job.items; i++) {
-    const int code = L.err[(size_t)i];
+    const int code = err[i];
     if (code == QM_ERR_POOL) fail(code, "The uniform pool is exhausted.");
     if (code == QM_ERR_MASS) fail(code, "The state has no finite positive probability mass.");
     if (code) fail(code, "batched state failed");
   }
-  if (samples && job.shots) std::memcpy(samples, L.samples.data(), L.samples.size() * 8);
-  if (outcomes && job.measure) std::memcpy(outcomes, L.outcomes.data(), L.outcomes.size() * 8);
-  if (consumed) std::memcpy(consumed, L.consumed.data(), L.consumed.size() * 8);
-  out.mass = L.mass;
+  out.mass.assign(mass, mass + job.items);
   out.numer.resize((size_t)(job.items * job.n_terms));
   for (int64_t i = 0; i < job.items; i++)
     for (int64_t k = 0; k < job.n_terms; k++)
-      out.numer[(size_t)(i * job.n_terms + g.order[(size_t)k])] = L.numer[(size_t)(i * job.n_terms + k)];
-  out.fallbacks = L.fallbacks;
-  out.device_ms = L.device_ms;
-  out.passes = (int)L.prog.passes.size();
+      out.numer[(size_t)(i * job.n_terms + g.order[(size_t)k])] = numer[i * job.n_terms + k];
+  const int64_t state_bytes = amp_bytes(job.precision) << job.qubits;
+  out.passes = passes;
   out.state_bytes = state_bytes;
   // Each pass reads and writes the state; the final stage reads it once per
   // mass, per Pauli group and for the CDF.
   const double finals = (job.want_mass || job.n_terms ? 1.0 : 0.0) + (double)g.groups.size() +
                         (job.shots || job.measure ? 1.0 : 0.0);
-  out.algorithmic_bytes =
-      (double)job.items * (double)state_bytes * (2.0 * (double)L.prog.passes.size() + finals);
+  out.algorithmic_bytes = (double)job.items * (double)state_bytes * (2.0 * (double)passes + finals);
+}
+
+void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes, int64_t *consumed,
+               bool timed, BatchOut &out) {
+  BatchLaunch L;
+  const TermGroups g = build_launch(job, L);
+  L.timed = timed;
+  run_batch_launch(c, L);
+  collect_out(job, g, L.err.data(), L.numer.data(), L.mass.data(), (int)L.prog.passes.size(), out);
+  if (samples && job.shots) std::memcpy(samples, L.samples.data(), L.samples.size() * 8);
+  if (outcomes && job.measure) std::memcpy(outcomes, L.outcomes.data(), L.outcomes.size() * 8);
+  if (consumed) std::memcpy(consumed, L.consumed.data(), L.consumed.size() * 8);
+  out.fallbacks = L.fallbacks;
+  out.device_ms = L.device_ms;
 }
 
 void fill_profile(qm_profile *p, const BatchOut &o) {
@@ -532,6 +571,193 @@ void check_parameters(const double *parameters, int64_t n_params) {
     if (!std::isfinite(parameters[i])) fail(QM_ERR_RANGE, "Parameters must be finite.");
 }
 
+// The 1 + 2R items of a parameter-shift gradient: item 0 the bound circuit,
+// items 1 + 2r / 2 + 2r rotation occurrence r shifted by +pi/2 / -pi/2
+// (VariationalEnergyEvaluator.cs:50-80).  The shift is folded into the
+// rotation's angle: R(a) R(s) = R(a + s) for one axis.
+BatchJob gradient_job(int qubits, int precision, const Template &tp, const qm_pauli *terms,
+                      int64_t n_terms) {
+  const int64_t R = (int64_t)tp.rot_kind.size();
+  BatchJob job;
+  job.qubits = qubits;
+  job.precision = precision;
+  job.steps = tp.steps;
+  job.items = 1 + 2 * R;
+  job.slots = (int32_t)std::max<int64_t>(R, 1);
+  job.angles.assign((size_t)(job.items * job.slots * 2), 0.0);
+  job.terms = terms;
+  job.n_terms = n_terms;
+  job.want_mass = true;
+  return job;
+}
+
+void gradient_angles(const Template &tp, const double *parameters, int32_t slots, double *angles) {
+  const int64_t R = (int64_t)tp.rot_kind.size();
+  for (int64_t i = 0; i < 1 + 2 * R; i++)
+    for (int64_t r = 0; r < R; r++) {
+      double a = parameters[tp.rot_param[(size_t)r]];
+      if (i > 0 && (i - 1) / 2 == r) a += (i % 2 == 1) ? M_PI / 2 : -M_PI / 2;
+      put_angle(&angles[(size_t)((i * slots + r) * 2)], a);
+    }
+}
+
+// n_sets parameter vectors, one item each.
+BatchJob energy_job(int qubits, int precision, const Template &tp, const qm_pauli *terms,
+                    int64_t n_terms, int64_t n_sets) {
+  BatchJob job;
+  job.qubits = qubits;
+  job.precision = precision;
+  job.steps = tp.steps;
+  job.items = n_sets;
+  job.slots = (int32_t)std::max<int64_t>((int64_t)tp.rot_kind.size(), 1);
+  job.angles.assign((size_t)(job.items * job.slots * 2), 0.0);
+  job.terms = terms;
+  job.n_terms = n_terms;
+  job.want_mass = true;
+  return job;
+}
+
+void energy_angles(const Template &tp, const double *sets, int64_t n_params, int64_t n_sets,
+                   int32_t slots, double *angles) {
+  const int64_t R = (int64_t)tp.rot_kind.size();
+  for (int64_t i = 0; i < n_sets; i++)
+    for (int64_t r = 0; r < R; r++)
+      put_angle(&angles[(size_t)((i * slots + r) * 2)], sets[i * n_params + tp.rot_param[(size_t)r]]);
+}
+
+// Host assembly in the reference's order (VariationalEnergyEvaluator.cs:50-86):
+// per rotation occurrence and term, c * (0.5 plus - 0.5 minus) Neumaier-summed
+// into its parameter, then the corrections added.
+void assemble_gradient(const BatchOut &out, const Template &tp, const double *coeffs, int64_t n_terms,
+                       int64_t n_params, double *energy, double *gradient) {
+  const int64_t R = (int64_t)tp.rot_kind.size();
+  *energy = energy_of(out, 0, coeffs, n_terms);
+  std::vector<double> grad((size_t)n_params, 0.0), corr((size_t)n_params, 0.0);
+  std::vector<double> plus((size_t)n_terms);
+  for (int64_t r = 0; r < R; r++) {
+    const int64_t param = tp.rot_param[(size_t)r];
+    const int64_t ip = 1 + 2 * r, im = 2 + 2 * r;
+    if (n_terms == 0 && !valid_mass(out.mass[(size_t)ip]))
+      fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+    for (int64_t t = 0; t < n_terms; t++) plus[(size_t)t] = expectation(out, ip, n_terms, t);
+    if (n_terms == 0 && !valid_mass(out.mass[(size_t)im]))
+      fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
+    for (int64_t t = 0; t < n_terms; t++) {
+      const double minus = expectation(out, im, n_terms, t);
+      const double contribution = coeffs[t] * (0.5 * plus[(size_t)t] - 0.5 * minus);
+      const double prev = grad[(size_t)param];
+      const double next = prev + contribution;
+      corr[(size_t)param] += std::fabs(prev) >= std::fabs(contribution) ? (prev - next) + contribution
+                                                                          : (contribution - next) + prev;
+      grad[(size_t)param] = next;
+      if (!std::isfinite(next) || !std::isfinite(corr[(size_t)param]))
+        fail(QM_ERR_NONFINITE, "The gradient accumulation is nonfinite.");
+    }
+  }
+  for (int64_t p = 0; p < n_params; p++) {
+    grad[(size_t)p] += corr[(size_t)p];
+    if (!std::isfinite(grad[(size_t)p])) fail(QM_ERR_NONFINITE, "The gradient is nonfinite.");
+    gradient[p] = grad[(size_t)p];
+  }
+}
+
+// One batched job captured as a hipGraph and replayed: the arena stays
+// allocated, the angle table is uploaded from pinned memory by the graph's
+// first node and the per-item outputs land in pinned memory.  Only the angles
+// change between replays.
+struct GraphedBatch {
+  Ctx *c = nullptr;
+  BatchJob job;
+  BatchLaunch L;
+  TermGroups g;
+  BatchArena A;
+  double *h_angles = nullptr;
+  char *h_out = nullptr;  // err | numer | mass | fallbacks
+  size_t off_numer = 0, off_mass = 0, off_fb = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~GraphedBatch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (h_angles) (void)hipHostFree(h_angles);
+    if (h_out) (void)hipHostFree(h_out);
+  }
+
+  // The stream work of one evaluation, issued eagerly or under capture.
+  void issue() {
+    hipStream_t st = c->stream;
+    const BatchArgs a = A.args(L);
+    QM_HIP(hipMemcpyAsync(A.ar.at<char>(A.o_angles), h_angles, L.angles.size() * 8,
+                          hipMemcpyHostToDevice, st));
+    k_zero_counter<<<1, 64, 0, st>>>(A.ar.at<unsigned long long>(A.o_fb));
+    QM_HIP(hipGetLastError());
+    int64_t grid = A.slots;
+    launch_batch(c, L.precision, L.shape, a, &grid);
+    const int64_t n = (int64_t)L.items.size();
+    QM_HIP(hipMemcpyAsync(h_out, A.ar.at<char>(A.o_err), (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    QM_HIP(hipMemcpyAsync(h_out + off_numer, A.ar.at<char>(A.o_numer), (size_t)L.n_numer * 8,
+                          hipMemcpyDeviceToHost, st));
+    QM_HIP(hipMemcpyAsync(h_out + off_mass, A.ar.at<char>(A.o_mass), (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    QM_HIP(hipMemcpyAsync(h_out + off_fb, A.ar.at<char>(A.o_fb), 8, hipMemcpyDeviceToHost, st));
+  }
+
+  void build(Ctx *ctx, BatchJob &&j) {
+    c = ctx;
+    job = std::move(j);
+    g = build_launch(job, L);
+    A.setup(c, L);
+    const int64_t n = (int64_t)L.items.size();
+    if (hipHostMalloc((void **)&h_angles, std::max<size_t>(L.angles.size(), 1) * 8) != hipSuccess ||
+        (off_numer = ((size_t)n * 4 + 7) & ~(size_t)7, off_mass = off_numer + (size_t)L.n_numer * 8,
+         off_fb = off_mass + (size_t)n * 8,
+         hipHostMalloc((void **)&h_out, off_fb + 8) != hipSuccess)) {
+      (void)hipGetLastError();
+      fail(QM_ERR_BUDGET, "pinned host memory for a graphed batch is not available");
+    }
+    std::memcpy(h_angles, L.angles.data(), L.angles.size() * 8);
+    // One eager evaluation first: it loads the kernel and sets its launch
+    // attributes outside the capture.
+    issue();
+    QM_HIP(hipStreamSynchronize(c->stream));
+    QM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    issue();
+    QM_HIP(hipStreamEndCapture(c->stream, &graph));
+    QM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+
+  // Replays with the angles already in h_angles; fills `out` like run_batch.
+  void replay(bool timed, BatchOut &out) {
+    hipStream_t st = c->stream;
+    if (timed) QM_HIP(hipEventRecord(c->ev_start, st));
+    QM_HIP(hipGraphLaunch(exec, st));
+    if (timed) QM_HIP(hipEventRecord(c->ev_stop, st));
+    QM_HIP(hipStreamSynchronize(st));
+    collect_out(job, g, reinterpret_cast<const int32_t *>(h_out),
+                reinterpret_cast<const double *>(h_out + off_numer),
+                reinterpret_cast<const double *>(h_out + off_mass), (int)L.prog.passes.size(), out);
+    out.fallbacks = (int64_t) * reinterpret_cast<const unsigned long long *>(h_out + off_fb);
+    if (timed) {
+      float ms = 0;
+      QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+      out.device_ms = ms;
+    }
+  }
+};
+
+// A variational program: one template and Hamiltonian, evaluated many times
+// (a VQE loop) through graphed batches.
+struct VqeProgram {
+  Ctx *c = nullptr;
+  int qubits = 0, precision = QM_C128;
+  int64_t n_params = 0;
+  Template tp;
+  std::vector<qm_pauli> terms;
+  std::vector<double> coeffs;
+  std::unique_ptr<GraphedBatch> grad;
+  std::map<int64_t, std::unique_ptr<GraphedBatch>> energy;  // by number of parameter sets
+};
+
 Ctx *batch_ctx(qm_ctx *h, qm_profile *profile) {
   Ctx *c = reinterpret_cast<Ctx *>(h);
   if (!c) fail(QM_ERR_ARG, "context is null");
@@ -641,58 +867,11 @@ int qm_energy_gradient_batch(qm_ctx *h, int32_t qubits, int32_t precision, const
     if (!energy || (n_params > 0 && !gradient)) fail(QM_ERR_ARG, "outputs are null");
     std::lock_guard<std::recursive_mutex> lk(c->lock);
     Template tp = make_template(qubits, ops, param_index, n_ops, n_params);
-    const int64_t R = (int64_t)tp.rot_kind.size();
-    // Item 0: the bound circuit; items 1 + 2r / 2 + 2r: rotation occurrence r
-    // shifted by +pi/2 / -pi/2 (VariationalEnergyEvaluator.cs:50-80).  The shift
-    // is folded into the rotation's angle: R(a) R(s) = R(a + s) for one axis.
-    BatchJob job;
-    job.qubits = qubits;
-    job.precision = precision;
-    job.steps = tp.steps;
-    job.items = 1 + 2 * R;
-    job.slots = (int32_t)std::max<int64_t>(R, 1);
-    job.angles.assign((size_t)(job.items * job.slots * 2), 0.0);
-    for (int64_t i = 0; i < job.items; i++)
-      for (int64_t r = 0; r < R; r++) {
-        double a = parameters[tp.rot_param[(size_t)r]];
-        if (i > 0 && (i - 1) / 2 == r) a += (i % 2 == 1) ? M_PI / 2 : -M_PI / 2;
-        put_angle(&job.angles[(size_t)((i * job.slots + r) * 2)], a);
-      }
-    job.terms = terms;
-    job.n_terms = n_terms;
-    job.want_mass = true;
+    BatchJob job = gradient_job(qubits, precision, tp, terms, n_terms);
+    gradient_angles(tp, parameters, job.slots, job.angles.data());
     BatchOut out;
     run_batch(c, job, nullptr, nullptr, nullptr, profile != nullptr, out);
-    // Host assembly in the reference's order (VariationalEnergyEvaluator.cs:50-86).
-    *energy = energy_of(out, 0, coeffs, n_terms);
-    std::vector<double> grad((size_t)n_params, 0.0), corr((size_t)n_params, 0.0);
-    std::vector<double> plus((size_t)n_terms);
-    for (int64_t r = 0; r < R; r++) {
-      const int64_t param = tp.rot_param[(size_t)r];
-      const int64_t ip = 1 + 2 * r, im = 2 + 2 * r;
-      if (n_terms == 0 && !valid_mass(out.mass[(size_t)ip]))
-        fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
-      for (int64_t t = 0; t < n_terms; t++) plus[(size_t)t] = expectation(out, ip, n_terms, t);
-      if (n_terms == 0 && !valid_mass(out.mass[(size_t)im]))
-        fail(QM_ERR_MASS, "The state has no finite positive probability mass.");
-      for (int64_t t = 0; t < n_terms; t++) {
-        const double minus = expectation(out, im, n_terms, t);
-        const double contribution = coeffs[t] * (0.5 * plus[(size_t)t] - 0.5 * minus);
-        const double prev = grad[(size_t)param];
-        const double next = prev + contribution;
-        corr[(size_t)param] += std::fabs(prev) >= std::fabs(contribution)
-                                   ? (prev - next) + contribution
-                                   : (contribution - next) + prev;
-        grad[(size_t)param] = next;
-        if (!std::isfinite(next) || !std::isfinite(corr[(size_t)param]))
-          fail(QM_ERR_NONFINITE, "The gradient accumulation is nonfinite.");
-      }
-    }
-    for (int64_t p = 0; p < n_params; p++) {
-      grad[(size_t)p] += corr[(size_t)p];
-      if (!std::isfinite(grad[(size_t)p])) fail(QM_ERR_NONFINITE, "The gradient is nonfinite.");
-      gradient[p] = grad[(size_t)p];
-    }
+    assemble_gradient(out, tp, coeffs, n_terms, n_params, energy, gradient);
     fill_profile(profile, out);
   });
 }
@@ -711,25 +890,105 @@ int qm_energy_batch(qm_ctx *h, int32_t qubits, int32_t precision, const qm_op *o
     if (!energies) fail(QM_ERR_ARG, "energies is null");
     std::lock_guard<std::recursive_mutex> lk(c->lock);
     Template tp = make_template(qubits, ops, param_index, n_ops, n_params);
-    const int64_t R = (int64_t)tp.rot_kind.size();
-    BatchJob job;
-    job.qubits = qubits;
-    job.precision = precision;
-    job.steps = tp.steps;
-    job.items = n_sets;
-    job.slots = (int32_t)std::max<int64_t>(R, 1);
-    job.angles.assign((size_t)(job.items * job.slots * 2), 0.0);
-    for (int64_t i = 0; i < n_sets; i++)
-      for (int64_t r = 0; r < R; r++)
-        put_angle(&job.angles[(size_t)((i * job.slots + r) * 2)],
-                  parameter_sets[i * n_params + tp.rot_param[(size_t)r]]);
-    job.terms = terms;
-    job.n_terms = n_terms;
-    job.want_mass = true;
+    BatchJob job = energy_job(qubits, precision, tp, terms, n_terms, n_sets);
+    energy_angles(tp, parameter_sets, n_params, n_sets, job.slots, job.angles.data());
     BatchOut out;
     run_batch(c, job, nullptr, nullptr, nullptr, profile != nullptr, out);
     for (int64_t i = 0; i < n_sets; i++) energies[i] = energy_of(out, i, coeffs, n_terms);
     fill_profile(profile, out);
+  });
+}
+
+int qm_vqe_program_create(qm_ctx *h, int32_t qubits, int32_t precision, const qm_op *ops,
+                          const int32_t *param_index, int64_t n_ops, int64_t n_params,
+                          const qm_pauli *terms, const double *coeffs, int64_t n_terms,
+                          qm_vqe_program **out) {
+  return guarded([&] {
+    Ctx *c = batch_ctx(h, nullptr);
+    if (!out) fail(QM_ERR_ARG, "out is null");
+    check_width(qubits, precision);
+    if (batch_shape(precision, qubits).K == 0)
+      fail(QM_ERR_UNSUPPORTED, "Batched execution needs at least 8 qubits.");
+    if (n_params < 0) fail(QM_ERR_RANGE, "parameterCount");
+    validate_terms(terms, n_terms, qubits);
+    if (n_terms > 0 && !coeffs) fail(QM_ERR_ARG, "coefficients are null");
+    auto p = std::make_unique<VqeProgram>();
+    p->c = c;
+    p->qubits = qubits;
+    p->precision = precision;
+    p->n_params = n_params;
+    p->tp = make_template(qubits, ops, param_index, n_ops, n_params);
+    p->terms.assign(terms, terms + n_terms);
+    p->coeffs.assign(coeffs, coeffs + n_terms);
+    *out = reinterpret_cast<qm_vqe_program *>(p.release());
+  });
+}
+
+int qm_vqe_program_gradient(qm_vqe_program *h, const double *parameters, double *energy,
+                            double *gradient, qm_profile *profile) {
+  return guarded([&] {
+    VqeProgram *p = reinterpret_cast<VqeProgram *>(h);
+    if (!p) fail(QM_ERR_ARG, "program is null");
+    batch_ctx(reinterpret_cast<qm_ctx *>(p->c), profile);
+    check_parameters(parameters, p->n_params);
+    if (!energy || (p->n_params > 0 && !gradient)) fail(QM_ERR_ARG, "outputs are null");
+    std::lock_guard<std::recursive_mutex> lk(p->c->lock);
+    const int64_t n_terms = (int64_t)p->terms.size();
+    if (!p->grad) {
+      auto gb = std::make_unique<GraphedBatch>();
+      BatchJob job = gradient_job(p->qubits, p->precision, p->tp, p->terms.data(), n_terms);
+      gradient_angles(p->tp, parameters, job.slots, job.angles.data());
+      gb->build(p->c, std::move(job));
+      p->grad = std::move(gb);
+    }
+    GraphedBatch &gb = *p->grad;
+    gradient_angles(p->tp, parameters, gb.job.slots, gb.h_angles);
+    BatchOut out;
+    gb.replay(profile != nullptr, out);
+    assemble_gradient(out, p->tp, p->coeffs.data(), n_terms, p->n_params, energy, gradient);
+    fill_profile(profile, out);
+  });
+}
+
+int qm_vqe_program_energies(qm_vqe_program *h, const double *parameter_sets, int64_t n_sets,
+                            double *energies, qm_profile *profile) {
+  return guarded([&] {
+    VqeProgram *p = reinterpret_cast<VqeProgram *>(h);
+    if (!p) fail(QM_ERR_ARG, "program is null");
+    batch_ctx(reinterpret_cast<qm_ctx *>(p->c), profile);
+    if (n_sets < 1) fail(QM_ERR_RANGE, "n_sets must be positive.");
+    check_parameters(parameter_sets, p->n_params * n_sets);
+    if (!energies) fail(QM_ERR_ARG, "energies is null");
+    std::lock_guard<std::recursive_mutex> lk(p->c->lock);
+    const int64_t n_terms = (int64_t)p->terms.size();
+    std::unique_ptr<GraphedBatch> &slot = p->energy[n_sets];
+    if (!slot) {
+      auto gb = std::make_unique<GraphedBatch>();
+      BatchJob job = energy_job(p->qubits, p->precision, p->tp, p->terms.data(), n_terms, n_sets);
+      energy_angles(p->tp, parameter_sets, p->n_params, n_sets, job.slots, job.angles.data());
+      gb->build(p->c, std::move(job));
+      slot = std::move(gb);
+    }
+    energy_angles(p->tp, parameter_sets, p->n_params, n_sets, slot->job.slots, slot->h_angles);
+    BatchOut out;
+    slot->replay(profile != nullptr, out);
+    for (int64_t i = 0; i < n_sets; i++) energies[i] = energy_of(out, i, p->coeffs.data(), n_terms);
+    fill_profile(profile, out);
+  });
+}
+
+int qm_vqe_program_destroy(qm_vqe_program *h) {
+  return guarded([&] {
+    VqeProgram *p = reinterpret_cast<VqeProgram *>(h);
+    if (!p) return;
+    use_device(p->c);
+    {
+      std::lock_guard<std::recursive_mutex> lk(p->c->lock);
+      QM_HIP(hipStreamSynchronize(p->c->stream));
+      p->grad.reset();
+      p->energy.clear();
+    }
+    delete p;
   });
 }
 

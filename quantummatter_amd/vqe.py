@@ -9,9 +9,10 @@ sufficient-decrease test compared in log space (VqeSolver.cs:103-110), with
 the same evaluation accounting and termination reasons.
 
 What runs on the device: every energy-and-gradient call is one launch of all
-1 + 2R parameter-shift circuits (``qm_energy_gradient_batch``) and every
-line-search energy one ``qm_energy_batch`` launch, for widths the batched
-kernel supports; narrower circuits use resident-state sessions.  The solver's
+1 + 2R parameter-shift circuits and every line-search energy one launch, for
+widths the batched kernel supports, both replayed from hipGraphs captured at
+their first use (``qm_vqe_program_*``); narrower circuits use resident-state
+sessions.  The solver's
 own arithmetic is a handful of host FP64 operations per parameter.
 
 Differences from the reference: the reference budgets host arrays
@@ -29,7 +30,7 @@ from dataclasses import dataclass, replace
 from typing import Optional, Sequence, Tuple
 
 from .backend import C128, CancellationToken, HipStateVectorBackend
-from .batched import HipBatchedEnergyEvaluator, batch_min_qubits
+from .batched import HipVqeProgram, batch_min_qubits
 from .circuit import ParameterizedCircuit, PauliHamiltonian
 from .errors import ArgumentError, ArgumentOutOfRangeError, OperationCanceledError
 from .variational import HipVariationalEnergyEvaluator
@@ -136,12 +137,15 @@ class HipVqeSolver:
         self._precision = precision
         self._evaluator = evaluator
 
-    def _evaluator_for(self, qubits: int):
+    def _evaluator_for(self, circuit: ParameterizedCircuit, hamiltonian: PauliHamiltonian):
+        """The injected evaluator; else a graphed program for batched widths (its
+        gradient and line-search launches are captured once and replayed), else
+        resident-state sessions.  Returns (evaluator, owned)."""
         if self._evaluator is not None:
-            return self._evaluator
-        if qubits >= batch_min_qubits(self._precision):
-            return HipBatchedEnergyEvaluator(self._backend, self._precision)
-        return HipVariationalEnergyEvaluator(self._backend, self._precision)
+            return self._evaluator, False
+        if circuit.QubitCount >= batch_min_qubits(self._precision):
+            return HipVqeProgram(self._backend, circuit, hamiltonian, self._precision), True
+        return HipVariationalEnergyEvaluator(self._backend, self._precision), False
 
     def Solve(self, circuit: ParameterizedCircuit, hamiltonian: PauliHamiltonian,
               initial_parameters: Sequence[float],
@@ -150,8 +154,15 @@ class HipVqeSolver:
         (VqeSolver.cs:27-117).  Cancellation and backend or numerical failures raise
         without a partial result."""
         self._validate(circuit, hamiltonian, initial_parameters, cancellation)
+        evaluator, owned = self._evaluator_for(circuit, hamiltonian)
+        try:
+            return self._solve(evaluator, circuit, hamiltonian, initial_parameters, cancellation)
+        finally:
+            if owned:
+                evaluator.Dispose()
+
+    def _solve(self, evaluator, circuit, hamiltonian, initial_parameters, cancellation) -> VqeResult:
         o = self._options
-        evaluator = self._evaluator_for(circuit.QubitCount)
         parameters = [float(v) for v in initial_parameters]
         trial = [0.0] * len(parameters)
         energy = float(evaluator.Evaluate(circuit, hamiltonian, parameters))

@@ -3,8 +3,8 @@
 The reference's solver cases (tests/QuantumMatter.Tests/VqeSolverTests.cs)
 run through the HIP session evaluator (1-2 qubits), with the same expected
 energies, parameters, counts and termination reasons; a 10-qubit
-hardware-efficient ansatz solved through the batched evaluator
-(qm_energy_gradient_batch / qm_energy_batch) follows the oracle-driven solve
+hardware-efficient ansatz solved through the graphed batched
+program (qm_vqe_program_*) follows the oracle-driven solve
 iterate for iterate: same termination reason and counts, energies within
 1e-10 and parameters within 1e-9.
 """
@@ -16,7 +16,7 @@ import vqe_cases as vc
 from hip_adapter import shared_backend
 from quantummatter_amd import workloads
 from quantummatter_amd.backend import CancellationToken
-from quantummatter_amd.batched import HipBatchedEnergyEvaluator, batch_min_qubits
+from quantummatter_amd.batched import HipVqeProgram, batch_min_qubits
 from quantummatter_amd.variational import HipVariationalEnergyEvaluator
 from quantummatter_amd.vqe import HipVqeSolver, VqeOptions
 
@@ -73,4 +73,6 @@ def test_batched_solve_follows_the_oracle_solve():
                                             want.GradientEvaluationCount, want.CircuitEvaluationCount)
     assert abs(got.Energy - want.Energy) <= 1e-10
     np.testing.assert_allclose(got.Parameters, want.Parameters, rtol=0, atol=1e-9)
-    assert isinstance(HipVqeSolver(shared_backend())._evaluator_for(n), HipBatchedEnergyEvaluator)
+    ev, owned = HipVqeSolver(shared_backend())._evaluator_for(circuit, ham)
+    assert isinstance(ev, HipVqeProgram) and owned
+    ev.Dispose()
