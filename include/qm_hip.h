@@ -260,6 +260,41 @@ int qm_jit_stats(int64_t *compiled, int64_t *cache_hits, double *compile_seconds
 int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n_ops,
                          int32_t tile_bits, int32_t lane_bits, int32_t *passes);
 
+/* One instruction of a dynamic program (src/QuantumMatter.Core/Dynamic/
+ * DynamicCircuit.cs:4-137, Dynamic/NoisyDynamicCircuit.cs:7-99). */
+#define QM_DYN_SEGMENT 0 /* ops[begin, begin+op_count), run when (classical & cond_mask) == cond_value */
+#define QM_DYN_MEASURE 1 /* measure sel_qubits[begin..begin+count) into sel_cbits[...] (one draw) */
+#define QM_DYN_RESET 2   /* measure `target` (one draw) and apply X on outcome 1; no classical write */
+#define QM_DYN_NOISE 3   /* channel noise_kind of `probability` on `target` (p == 0: nothing) */
+typedef struct qm_dyn_step {
+  int32_t kind;
+  int32_t cond_mask;  /* 0 = unconditional */
+  int32_t cond_value;
+  int32_t target;
+  int32_t noise_kind;
+  int32_t count;
+  int64_t begin;
+  int64_t op_count;
+  double probability;
+} qm_dyn_step;
+
+/* StateVector.EvolveDynamic / EvolveNoisyDynamic (src/QuantumMatter.Core/
+ * StateVector.Dynamic.cs:11-25, StateVector.NoisyDynamic.cs:11-27) via
+ * DynamicCircuit.Execute (Dynamic/DynamicCircuit.cs:77-112) and
+ * NoisyDynamicCircuit.Execute (Dynamic/NoisyDynamicCircuit.cs:79-97) on a
+ * resident state: the classical register (classical_bits wide) starts at 0;
+ * draws come from u[*cursor...] in the reference's order (one per Measure,
+ * one per Reset, the channel rules of qm_state_noise).  Every instruction is
+ * validated before the state is touched.  The state never leaves the device;
+ * between measurements, segments run as one gate sequence under `schedule`.
+ * On failure or cancellation the state may be partly evolved and *classical
+ * is not written. */
+int qm_state_dynamic_execute(qm_state *state, int32_t classical_bits, const qm_dyn_step *steps,
+                             int64_t n_steps, const qm_op *ops, int64_t n_ops,
+                             const int32_t *sel_qubits, const int32_t *sel_cbits, int64_t n_sel,
+                             int32_t schedule, const double *uniforms, int64_t n_uniforms,
+                             int64_t *cursor, int32_t *classical, const int32_t *cancel);
+
 /* ---------------------------------------------------------------------------
  * Batched execution: one program over many same-width states, one kernel
  * launch, no host round trip between the first gate and the last result.

@@ -51,6 +51,7 @@ def lib():
             "qmo_estimate_probabilities": [I32, P, I64, P, I64, P, I64, P, P],
             "qmo_bind": [P, P, I64, P, I64, I64, F64, P, P],
             "qmo_energy_gradient": [I32, P, P, I64, P, I64, P, P, I64, P, P],
+            "qmo_dynamic_execute": [I32, P, I64, P, P, P, P, P, I64, P, P],
             "qmo_energy_bound": [I32, P, P, I64, P, I64, P, P, I64, P],
             "qmo_parallel_execute": [I32, P, I64, P, I32, I32, I32, I32],
         }
@@ -227,3 +228,17 @@ def energy_bound(n, ops, param_index, parameters, paulis, coeffs):
     _check(lib().qmo_energy_bound(n, _p(ops), _p(param_index), len(ops), _p(prm), len(prm),
                                   _p(paulis), _p(c), len(c), _p(e)), "energy_bound")
     return float(e[0])
+
+
+def dynamic_execute(n, steps, ops, sel_q, sel_c, uniforms, state=None):
+    """DynamicCircuit.Execute / NoisyDynamicCircuit.Execute on a copy of `state`
+    (|0..0> if None); returns (state, classical register, draws consumed)."""
+    s = zero_state(n) if state is None else np.array(state, dtype=np.complex128, copy=True)
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    cursor = np.zeros(1, dtype=np.int64)
+    reg = np.zeros(1, dtype=np.int32)
+    sq = np.ascontiguousarray(sel_q, dtype=np.int32)
+    sc = np.ascontiguousarray(sel_c, dtype=np.int32)
+    _check(lib().qmo_dynamic_execute(n, _p(steps), len(steps), _p(np.ascontiguousarray(ops)), _p(sq), _p(sc),
+                                     _p(s), _p(u), len(u), _p(cursor), _p(reg)), "dynamic_execute")
+    return s, int(reg[0]), int(cursor[0])

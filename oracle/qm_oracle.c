@@ -770,3 +770,62 @@ int qmo_energy_gradient(int qubits, const qmo_op *ops, const int32_t *param_inde
   free(plus);
   return rc;
 }
+
+/* ---------------------------------------------------------------------------
+ * Dynamic programs: Dynamic/DynamicCircuit.cs and Dynamic/NoisyDynamicCircuit.cs.
+ * ------------------------------------------------------------------------- */
+int qmo_dynamic_execute(int qubits, const qmo_dyn_step *steps, int64_t n_steps, const qmo_op *ops,
+                        const int32_t *sel_q, const int32_t *sel_c, double *amps, const double *u,
+                        int64_t n_u, int64_t *cursor, int32_t *classical) {
+  int32_t reg = 0; /* DynamicCircuit.cs:81: each execution starts at zero */
+  for (int64_t k = 0; k < n_steps; k++) {
+    const qmo_dyn_step *st = &steps[k];
+    int rc = QMO_OK;
+    double draw_u;
+    int64_t outcome;
+    switch (st->kind) {
+      case QMO_DYN_SEGMENT: /* :88-90 */
+        if ((reg & st->cond_mask) == st->cond_value)
+          rc = qmo_execute(qubits, ops + st->begin, st->op_count, amps);
+        break;
+      case QMO_DYN_MEASURE: { /* :91-100 */
+        rc = draw(u, n_u, cursor, &draw_u);
+        if (rc) break;
+        rc = qmo_measure(qubits, amps, sel_q + st->begin, st->count, draw_u, &outcome);
+        if (rc) break;
+        int32_t written = 0, write_mask = 0;
+        for (int b = 0; b < st->count; b++) {
+          written |= (int32_t)((outcome >> b) & 1) << sel_c[st->begin + b];
+          write_mask |= (int32_t)1 << sel_c[st->begin + b];
+        }
+        reg = (reg & ~write_mask) | written;
+        break;
+      }
+      case QMO_DYN_RESET: { /* :101-107 */
+        rc = draw(u, n_u, cursor, &draw_u);
+        if (rc) break;
+        const int32_t q = st->target;
+        rc = qmo_measure(qubits, amps, &q, 1, draw_u, &outcome);
+        if (rc || outcome == 0) break;
+        qmo_op x;
+        memset(&x, 0, sizeof x);
+        x.kind = QMO_X;
+        x.target = q;
+        x.control = -1;
+        x.second_control = -1;
+        rc = qmo_execute(qubits, &x, 1, amps);
+        break;
+      }
+      case QMO_DYN_NOISE: /* NoisyCircuit.cs:58: zero strength is skipped */
+        if (st->probability != 0)
+          rc = qmo_noise_apply(st->noise_kind, st->probability, st->target, qubits, amps, u, n_u,
+                               cursor);
+        break;
+      default:
+        rc = QMO_ERR_ARG;
+    }
+    if (rc) return rc;
+  }
+  *classical = reg;
+  return QMO_OK;
+}

@@ -164,6 +164,34 @@ int qmo_energy_bound(int qubits, const qmo_op *ops, const int32_t *param_index, 
                      const double *parameters, int64_t n_params, const qmo_pauli *terms,
                      const double *coeffs, int64_t n_terms, double *energy);
 
+/* One instruction of a dynamic program (Dynamic/DynamicCircuit.cs:4-137,
+ * Dynamic/NoisyDynamicCircuit.cs:7-99).  Layout is identical to qm_dyn_step
+ * in include/qm_hip.h. */
+enum { QMO_DYN_SEGMENT = 0, QMO_DYN_MEASURE = 1, QMO_DYN_RESET = 2, QMO_DYN_NOISE = 3 };
+typedef struct {
+  int32_t kind;
+  int32_t cond_mask;   /* segment: runs when (classical & cond_mask) == cond_value */
+  int32_t cond_value;
+  int32_t target;      /* reset qubit / noise target */
+  int32_t noise_kind;
+  int32_t count;       /* measure: selection size */
+  int64_t begin;       /* segment: first op; measure: first selection entry */
+  int64_t op_count;    /* segment: op count */
+  double probability;  /* noise */
+} qmo_dyn_step;
+
+/* DynamicCircuit.Execute (Dynamic/DynamicCircuit.cs:77-112) with the channel
+ * placements of NoisyDynamicCircuit.Execute (Dynamic/NoisyDynamicCircuit.cs:
+ * 79-97) inline: the classical register starts at 0; a segment runs when its
+ * condition holds; Measure draws once (StateVector.Observations.cs:45-102)
+ * and overwrites its destination bits; Reset measures its qubit (one draw)
+ * and applies X on outcome 1; a channel applies CpuNoiseChannelBackend's
+ * rules (p == 0: nothing).  Draws come from uniforms[*cursor...]. */
+int qmo_dynamic_execute(int qubits, const qmo_dyn_step *steps, int64_t n_steps, const qmo_op *ops,
+                        const int32_t *sel_qubits, const int32_t *sel_cbits, double *amps,
+                        const double *uniforms, int64_t n_uniforms, int64_t *cursor,
+                        int32_t *classical);
+
 #ifdef __cplusplus
 }
 #endif

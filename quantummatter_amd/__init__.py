@@ -8,14 +8,15 @@ workloads (``workloads``) and the ctypes binding of the library (``backend``).
 from .errors import (ArgumentError, ArgumentOutOfRangeError, DeviceError, InvalidOperationError,
                      ObjectDisposedError, OperationCanceledError, PlatformNotSupportedError,
                      PoolExhaustedError)
-from .circuit import (GateKind, GateOperation, NoiseChannel, NoiseKind, NoisyCircuit,
-                      ParameterizedCircuit, PauliHamiltonian, PauliString, PauliTerm, QuantumCircuit,
-                      SingleQubitUnitary, OP_DTYPE, PAULI_DTYPE, STEP_DTYPE)
+from .circuit import (DynamicCircuit, GateKind, GateOperation, NoiseChannel, NoiseKind, NoisyCircuit,
+                      NoisyDynamicCircuit, ParameterizedCircuit, PauliHamiltonian, PauliString, PauliTerm,
+                      QuantumCircuit, SingleQubitUnitary, OP_DTYPE, PAULI_DTYPE, STEP_DTYPE)
 
 __all__ = [
     "ArgumentError", "ArgumentOutOfRangeError", "DeviceError", "InvalidOperationError",
     "ObjectDisposedError", "OperationCanceledError", "PlatformNotSupportedError", "PoolExhaustedError",
-    "GateKind", "GateOperation", "NoiseChannel", "NoiseKind", "NoisyCircuit", "ParameterizedCircuit",
+    "DynamicCircuit", "NoisyDynamicCircuit", "GateKind", "GateOperation", "NoiseChannel", "NoiseKind",
+    "NoisyCircuit", "ParameterizedCircuit",
     "PauliHamiltonian", "PauliString", "PauliTerm", "QuantumCircuit", "SingleQubitUnitary",
     "OP_DTYPE", "PAULI_DTYPE", "STEP_DTYPE",
 ]

@@ -75,6 +75,7 @@ SIGNATURES = {
     "qm_batch_min_qubits": [I32, P],
     "qm_trajectories_run": [P, I32, I32, P, I64, P, I64, I64, P, I64, I64, P, I32, P, P, P, P, P],
     "qm_trajectories_estimate": [P, I32, I32, P, I64, P, I64, I64, P, I64, P, P, P, P, P],
+    "qm_state_dynamic_execute": [P, I32, P, I64, P, I64, P, P, I64, I32, P, I64, P, P, P],
     "qm_energy_gradient_batch": [P, I32, I32, P, P, I64, P, I64, P, P, I64, P, P, P],
     "qm_energy_batch": [P, I32, I32, P, P, I64, P, I64, I64, P, P, I64, P, P],
     "qm_vqe_program_create": [P, I32, I32, P, P, I64, I64, P, P, I64, P],

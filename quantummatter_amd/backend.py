@@ -248,6 +248,24 @@ class HipStateSession:
         return HipExecutionProfile(prof.device_ms, prof.state_bytes, prof.kernel_launches, prof.passes,
                                    prof.algorithmic_bytes)
 
+    def EvolveDynamic(self, circuit, uniforms: np.ndarray, cursor: int = 0, schedule: int = SCHED_FUSED,
+                      cancellation: Optional[CancellationToken] = None):
+        """StateVector.EvolveDynamic / EvolveNoisyDynamic (StateVector.Dynamic.cs:11-25,
+        StateVector.NoisyDynamic.cs:11-27) for a DynamicCircuit or NoisyDynamicCircuit:
+        draws come from uniforms[cursor...]; returns (classical register, advanced cursor)."""
+        if circuit is None:
+            raise ArgumentError("circuit is null.")
+        if circuit.QubitCount != self.QubitCount:
+            raise ArgumentError("Circuit and state sizes must match.")
+        steps, ops, sel_q, sel_c = circuit.packed()
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        cur = np.array([cursor], dtype=np.int64)
+        reg = np.zeros(1, dtype=np.int32)
+        call("qm_state_dynamic_execute", self._handle(), circuit.ClassicalBitCount, ptr(steps), len(steps),
+             ptr(ops), len(ops), ptr(sel_q), ptr(sel_c), len(sel_q), schedule, ptr(u), len(u), ptr(cur),
+             ptr(reg), _cancel_ptr(cancellation))
+        return int(reg[0]), int(cur[0])
+
     def ApplyNoise(self, channel: NoiseChannel, target: int, uniforms: np.ndarray, cursor: int = 0) -> int:
         """INoiseChannelBackend.Apply; returns the advanced cursor."""
         u = np.ascontiguousarray(uniforms, dtype=np.float64)

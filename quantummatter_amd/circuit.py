@@ -36,7 +36,13 @@ PAULI_DTYPE = np.dtype(
 STEP_DTYPE = np.dtype(
     [("is_channel", "<i4"), ("noise_kind", "<i4"), ("target", "<i4"), ("pad", "<i4"),
      ("probability", "<f8"), ("op_begin", "<i8"), ("op_count", "<i8")], align=True)
+DYN_STEP_DTYPE = np.dtype(
+    [("kind", "<i4"), ("cond_mask", "<i4"), ("cond_value", "<i4"), ("target", "<i4"),
+     ("noise_kind", "<i4"), ("count", "<i4"), ("begin", "<i8"), ("op_count", "<i8"),
+     ("probability", "<f8")], align=True)
 assert OP_DTYPE.itemsize == 88 and PAULI_DTYPE.itemsize == 24 and STEP_DTYPE.itemsize == 40
+assert DYN_STEP_DTYPE.itemsize == 48
+DYN_SEGMENT, DYN_MEASURE, DYN_RESET, DYN_NOISE = 0, 1, 2, 3
 
 #: Widest register the C ABI accepts (64-bit indexing; the reference stops at 30).
 MAX_QUBITS = 34
@@ -449,3 +455,114 @@ class ParameterizedCircuit:
     def packed(self):
         """Returns (template ops, parameter index per op) in qm_op / int32 layout."""
         return ops_array(self._ops), np.asarray(self._param_index, dtype=np.int32)
+
+
+class DynamicCircuit:
+    """Unitary snapshots, mid-circuit measurements, classical conditions and resets
+    (Dynamic/DynamicCircuit.cs:4-137): builder validation as the reference's, each
+    instruction copied when appended."""
+
+    def __init__(self, qubits: int, classical_bits: int):
+        if qubits < 1 or qubits > MAX_QUBITS:
+            raise ArgumentOutOfRangeError("qubits")
+        if classical_bits < 1 or classical_bits > 30:
+            raise ArgumentOutOfRangeError("classicalBits")
+        self.QubitCount = qubits
+        self.ClassicalBitCount = classical_bits
+        self._steps: list = []
+
+    @property
+    def StepCount(self) -> int:
+        return len(self._steps)
+
+    def _circuit(self, circuit: QuantumCircuit):
+        if circuit is None:
+            raise ArgumentError("circuit is null.")
+        if circuit.QubitCount != self.QubitCount:
+            raise ArgumentError("Circuit sizes must match.")
+        return tuple(circuit.Operations)
+
+    def Append(self, circuit: QuantumCircuit) -> "DynamicCircuit":
+        self._steps.append(("segment", self._circuit(circuit), 0, 0))
+        return self
+
+    def If(self, classical_mask: int, expected_value: int, circuit: QuantumCircuit) -> "DynamicCircuit":
+        ops = self._circuit(circuit)
+        register = (1 << self.ClassicalBitCount) - 1
+        if classical_mask <= 0 or classical_mask & ~register:
+            raise ArgumentOutOfRangeError("classicalMask")
+        if expected_value < 0 or expected_value & ~classical_mask:
+            raise ArgumentOutOfRangeError("expectedValue")
+        self._steps.append(("segment", ops, int(classical_mask), int(expected_value)))
+        return self
+
+    @staticmethod
+    def _indices(indices, count, name) -> None:
+        seen = set()
+        for i in indices:
+            if i < 0 or i >= count:
+                raise ArgumentOutOfRangeError(name)
+            if i in seen:
+                raise ArgumentError("Indices must be distinct.")
+            seen.add(i)
+
+    def Measure(self, qubits: Sequence[int], classical_bits: Sequence[int]) -> "DynamicCircuit":
+        qubits, classical_bits = list(qubits), list(classical_bits)
+        if not qubits or len(qubits) != len(classical_bits) or len(qubits) > self.QubitCount:
+            raise ArgumentError("Measurement requires equally sized nonempty selections.")
+        self._indices(qubits, self.QubitCount, "qubits")
+        self._indices(classical_bits, self.ClassicalBitCount, "classicalBits")
+        self._steps.append(("measure", tuple(qubits), tuple(classical_bits)))
+        return self
+
+    def Reset(self, qubit: int) -> "DynamicCircuit":
+        if qubit < 0 or qubit >= self.QubitCount:
+            raise ArgumentOutOfRangeError("qubit")
+        self._steps.append(("reset", int(qubit)))
+        return self
+
+    def _noise(self, channel: "NoiseChannel", target: int) -> None:
+        if channel is None:
+            raise ArgumentError("channel is null.")
+        if target < 0 or target >= self.QubitCount:
+            raise ArgumentOutOfRangeError("target")
+        self._steps.append(("noise", channel, int(target)))
+
+    def packed(self):
+        """(steps, ops, sel_qubits, sel_cbits) in the qm_dyn_step / qm_op layouts."""
+        steps = np.zeros(len(self._steps), dtype=DYN_STEP_DTYPE)
+        ops: List[GateOperation] = []
+        sel_q: List[int] = []
+        sel_c: List[int] = []
+        for i, st in enumerate(self._steps):
+            rec = steps[i]
+            if st[0] == "segment":
+                rec["kind"] = DYN_SEGMENT
+                rec["begin"], rec["op_count"] = len(ops), len(st[1])
+                rec["cond_mask"], rec["cond_value"] = st[2], st[3]
+                ops.extend(st[1])
+            elif st[0] == "measure":
+                rec["kind"] = DYN_MEASURE
+                rec["begin"], rec["count"] = len(sel_q), len(st[1])
+                sel_q.extend(st[1])
+                sel_c.extend(st[2])
+            elif st[0] == "reset":
+                rec["kind"] = DYN_RESET
+                rec["target"] = st[1]
+            else:
+                rec["kind"] = DYN_NOISE
+                rec["noise_kind"] = int(st[1].Kind)
+                rec["probability"] = st[1].Probability
+                rec["target"] = st[2]
+        return (steps, ops_array(ops), np.array(sel_q, dtype=np.int32), np.array(sel_c, dtype=np.int32))
+
+
+class NoisyDynamicCircuit(DynamicCircuit):
+    """A dynamic program with explicit single-qubit channel placements between
+    instructions (Dynamic/NoisyDynamicCircuit.cs:7-99).  A placement is
+    unconditional and leaves classical memory alone; zero strength consumes no
+    draw."""
+
+    def ApplyNoise(self, channel: "NoiseChannel", target: int) -> "NoisyDynamicCircuit":
+        self._noise(channel, target)
+        return self

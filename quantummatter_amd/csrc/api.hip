@@ -195,6 +195,7 @@ static State *as_state(qm_state *h) {
 
 static void noise_apply(State *s, int kind, double p, int target, const double *u, int64_t n_u,
                         int64_t *cursor);
+static double take(const double *u, int64_t n, int64_t *cursor);
 
 }  // namespace qm
 
@@ -570,22 +571,30 @@ static uint64_t selection_mask(const State *s, const int32_t *sel, int k) {
   return m;
 }
 
+// StateVector.Measure (StateVector.Observations.cs:45-102): one full-basis
+// draw with the sequential-CDF rule, the matching branch kept and divided by
+// the square root of its mass.  Returns the packed outcome (bit b = sel[b]).
+static int64_t measure_subset(State *s, const int32_t *sel, int k, double u) {
+  const uint64_t mask = selection_mask(s, sel, k);
+  int64_t basis = -1;
+  s->boundary_fallbacks += cdf_search(s, &u, 1, &basis, s->ctx->stream);
+  const uint64_t pattern = (uint64_t)basis & mask;
+  const double branch = reduce_masked_mass(s, mask, pattern, s->ctx->stream);
+  validate_mass(branch);
+  launch_collapse_subset(s, mask, pattern, std::sqrt(branch), s->ctx->stream);
+  int64_t out = 0;
+  for (int b = 0; b < k; b++) out |= (((uint64_t)basis >> sel[b]) & 1) << b;
+  return out;
+}
+
 int qm_state_measure(qm_state *h, const int32_t *sel, int32_t k, double u, int64_t *outcome) {
   return guarded([&] {
     State *s = as_state(h);
     if (!outcome) fail(QM_ERR_ARG, "outcome is null");
-    const uint64_t mask = selection_mask(s, sel, k);
+    selection_mask(s, sel, k);
     std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-    int64_t basis = -1;
-    s->boundary_fallbacks += cdf_search(s, &u, 1, &basis, s->ctx->stream);
-    const uint64_t pattern = (uint64_t)basis & mask;
-    const double branch = reduce_masked_mass(s, mask, pattern, s->ctx->stream);
-    validate_mass(branch);
-    const double norm = std::sqrt(branch);
-    launch_collapse_subset(s, mask, pattern, norm, s->ctx->stream);
+    const int64_t out = measure_subset(s, sel, k, u);
     QM_HIP(hipStreamSynchronize(s->ctx->stream));
-    int64_t out = 0;
-    for (int b = 0; b < k; b++) out |= (((uint64_t)basis >> sel[b]) & 1) << b;
     *outcome = out;
   });
 }
@@ -958,6 +967,112 @@ int qm_state_noisy_execute(qm_state *h, const qm_step *steps, int64_t n_steps, c
     }
     flush();
     QM_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int qm_state_dynamic_execute(qm_state *h, int32_t classical_bits, const qm_dyn_step *steps,
+                             int64_t n_steps, const qm_op *ops, int64_t n_ops,
+                             const int32_t *sel_qubits, const int32_t *sel_cbits, int64_t n_sel,
+                             int32_t schedule, const double *u, int64_t n_u, int64_t *cursor,
+                             int32_t *classical, const int32_t *cancel) {
+  return guarded([&] {
+    State *s = as_state(h);
+    if (!cursor || !classical || n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "buffers are null");
+    if (classical_bits < 1 || classical_bits > 30) fail(QM_ERR_RANGE, "classicalBits");
+    if (schedule < QM_SCHED_PER_GATE || schedule > QM_SCHED_COMPILED) fail(QM_ERR_ARG, "unknown schedule");
+    const int n = s->qubits;
+    const int32_t reg_mask = (int32_t)((1u << classical_bits) - 1);
+    // Every instruction is validated before any device work, as the builders
+    // (DynamicCircuit.cs:27-75, NoisyDynamicCircuit.cs:64-77) validate on append.
+    std::vector<std::vector<DevGate>> seg((size_t)n_steps);
+    for (int64_t k = 0; k < n_steps; k++) {
+      const qm_dyn_step &st = steps[k];
+      switch (st.kind) {
+        case QM_DYN_SEGMENT:
+          if (st.begin < 0 || st.op_count < 0 || st.begin + st.op_count > n_ops || (st.op_count > 0 && !ops))
+            fail(QM_ERR_ARG, "bad segment");
+          if (st.cond_mask < 0 || (st.cond_mask & ~reg_mask)) fail(QM_ERR_RANGE, "classicalMask");
+          if (st.cond_value < 0 || (st.cond_value & ~st.cond_mask)) fail(QM_ERR_RANGE, "expectedValue");
+          seg[(size_t)k] = prepare_all(ops + st.begin, st.op_count, n);
+          break;
+        case QM_DYN_MEASURE: {
+          if (st.count < 1 || st.count > n || st.begin < 0 || st.begin + st.count > n_sel || !sel_qubits ||
+              !sel_cbits)
+            fail(QM_ERR_ARG, "Measurement requires equally sized nonempty selections.");
+          selection_mask(s, sel_qubits + st.begin, st.count);
+          int32_t seen = 0;
+          for (int b = 0; b < st.count; b++) {
+            const int32_t c = sel_cbits[st.begin + b];
+            if (c < 0 || c >= classical_bits) fail(QM_ERR_RANGE, "classicalBits");
+            if (seen & (1 << c)) fail(QM_ERR_ARG, "Indices must be distinct.");
+            seen |= 1 << c;
+          }
+          break;
+        }
+        case QM_DYN_RESET:
+          if (st.target < 0 || st.target >= n) fail(QM_ERR_RANGE, "qubit");
+          break;
+        case QM_DYN_NOISE:
+          if (st.target < 0 || st.target >= n) fail(QM_ERR_RANGE, "target");
+          if (!std::isfinite(st.probability) || st.probability < 0 || st.probability > 1)
+            fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
+          if (st.noise_kind < 0 || st.noise_kind > 3) fail(QM_ERR_ARG, "unknown noise kind");
+          break;
+        default:
+          fail(QM_ERR_ARG, "unknown dynamic instruction");
+      }
+    }
+    check_cancel(cancel);
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    // DynamicCircuit.Execute (DynamicCircuit.cs:77-112) with NoisyDynamicCircuit's
+    // channel placements inline (NoisyDynamicCircuit.cs:79-97).  The state stays
+    // on the device throughout; the register lives on the host, where each
+    // measurement's outcome (one word) arrives.  Segments that run between two
+    // measurements, resets or channels are executed as one gate sequence.
+    int32_t reg = 0;
+    std::vector<DevGate> run;
+    auto flush = [&] {
+      if (!run.empty()) run_gates(s, run, schedule, cancel);
+      run.clear();
+    };
+    for (int64_t k = 0; k < n_steps; k++) {
+      check_cancel(cancel);
+      const qm_dyn_step &st = steps[k];
+      if (st.kind == QM_DYN_SEGMENT) {
+        if ((reg & st.cond_mask) == st.cond_value)
+          run.insert(run.end(), seg[(size_t)k].begin(), seg[(size_t)k].end());
+      } else if (st.kind == QM_DYN_MEASURE) {
+        flush();
+        const double draw = take(u, n_u, cursor);
+        const int64_t out = measure_subset(s, sel_qubits + st.begin, st.count, draw);
+        int32_t written = 0, write_mask = 0;
+        for (int b = 0; b < st.count; b++) {
+          written |= (int32_t)((out >> b) & 1) << sel_cbits[st.begin + b];
+          write_mask |= 1 << sel_cbits[st.begin + b];
+        }
+        reg = (reg & ~write_mask) | written;
+      } else if (st.kind == QM_DYN_RESET) {
+        flush();
+        const double draw = take(u, n_u, cursor);
+        const int32_t q = st.target;
+        if (measure_subset(s, &q, 1, draw) != 0) {
+          qm_op x{};
+          x.kind = 1;  // GateKind.X
+          x.target = q;
+          x.control = -1;
+          x.second_control = -1;
+          std::vector<DevGate> g2 = prepare_all(&x, 1, n);
+          run.insert(run.end(), g2.begin(), g2.end());
+        }
+      } else if (st.probability != 0) {
+        flush();
+        noise_apply(s, st.noise_kind, st.probability, st.target, u, n_u, cursor);
+      }
+    }
+    flush();
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+    check_cancel(cancel);
+    *classical = reg;
   });
 }
 
