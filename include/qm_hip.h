@@ -154,6 +154,8 @@ typedef struct qm_tune_entry {
   int32_t passes;
   double ms;
   double ms_per_gate;
+  int32_t contiguous;  /* lowest index bits kept in every tile */
+  int32_t remap;       /* 1: passes reassign qubits to those bits (compiled schedule) */
 } qm_tune_entry;
 int qm_autotune(qm_ctx *ctx, int32_t qubits, int32_t precision, int32_t schedule, const qm_op *ops,
                 int64_t n_ops, qm_tune_entry *entries, int32_t capacity, int32_t *count,
@@ -230,9 +232,11 @@ int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n
                   int32_t *passes, int32_t *transposes, int64_t *gates);
 /* Runs the fused schedule for an arbitrary tile shape on the host with the
  * kernels' exact arithmetic and indexing (complex128, qubits <= 26), so the
- * planner is verified without a device. */
+ * planner is verified without a device.  remap = 1 lets passes move qubits
+ * into and out of the `contiguous` lowest index bits (the state is back in
+ * natural order at the end). */
 int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
-                       int32_t lane_bits, int32_t contiguous, double *amplitudes);
+                       int32_t lane_bits, int32_t contiguous, int32_t remap, double *amplitudes);
 
 /* Decisions on CDF and branch boundaries: draws within the rigorous rounding
  * margin of a boundary were re-decided with the reference's sequential

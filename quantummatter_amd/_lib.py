@@ -25,7 +25,8 @@ class QmDeviceInfo(C.Structure):
 
 class QmTuneEntry(C.Structure):
     _fields_ = [("tile_bits", C.c_int32), ("lane_bits", C.c_int32), ("variant", C.c_int32),
-                ("passes", C.c_int32), ("ms", C.c_double), ("ms_per_gate", C.c_double)]
+                ("passes", C.c_int32), ("ms", C.c_double), ("ms_per_gate", C.c_double),
+                ("contiguous", C.c_int32), ("remap", C.c_int32)]
 
 
 class QmCtxOptions(C.Structure):
@@ -71,7 +72,7 @@ SIGNATURES = {
     "qm_state_inner": [P, P, P, P],
     "qm_state_boundary_fallbacks": [P, P],
     "qm_plan_fused": [I32, I32, P, I64, P, P, P],
-    "qm_plan_apply_host": [I32, P, I64, I32, I32, I32, P],
+    "qm_plan_apply_host": [I32, P, I64, I32, I32, I32, I32, P],
     "qm_batch_min_qubits": [I32, P],
     "qm_trajectories_run": [P, I32, I32, P, I64, P, I64, I64, P, I64, I64, P, I32, P, P, P, P, P],
     "qm_trajectories_estimate": [P, I32, I32, P, I64, P, I64, I64, P, I64, P, P, P, P, P],

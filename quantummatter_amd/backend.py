@@ -375,8 +375,9 @@ def dense_apply_host(n: int, ops: np.ndarray, state: np.ndarray, max_block_qubit
 
 
 def plan_apply_host(qubits: int, ops: np.ndarray, state: np.ndarray, tile_bits: int, lane_bits: int,
-                    contiguous: int) -> np.ndarray:
+                    contiguous: int, remap: bool = False) -> np.ndarray:
     """Runs the fused schedule on the host with the kernels' arithmetic (CPU verification)."""
     s = np.array(state, dtype=np.complex128, copy=True)
-    call("qm_plan_apply_host", qubits, ptr(ops), len(ops), tile_bits, lane_bits, contiguous, ptr(s))
+    call("qm_plan_apply_host", qubits, ptr(ops), len(ops), tile_bits, lane_bits, contiguous, int(remap),
+         ptr(s))
     return s

@@ -126,7 +126,28 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
   } else if (tuned_slot >= 0 && c->tuned[tuned_slot][s->precision][s->qubits].K) {
     shape = c->tuned[tuned_slot][s->precision][s->qubits];
   } else if (schedule == QM_SCHED_COMPILED && s->qubits < shape.K && s->qubits >= 12) {
-    shape = FusedShape{12, 8, 4, shape.contiguous};  // narrower states: 4096-amplitude tiles
+    const FusedShape wide = shape;
+    shape = FusedShape{12, 8, 4, wide.contiguous};  // narrower states: 4096-amplitude tiles
+    shape.remap = wide.remap;
+  }
+  if (!force && schedule == QM_SCHED_FUSED && !(tuned_slot >= 0 && c->tuned[0][s->precision][s->qubits].K)) {
+    // The interpreting kernel keeps the lowest qubits in place (its passes are
+    // bound by the kernel's own work, not by the access pattern).
+    shape.contiguous = s->precision == QM_C128 ? 3 : 4;
+    shape.remap = 0;
+  }
+  if (!force) {
+    // Tuning knob: lowest qubits kept in every tile (longer contiguous runs).
+    static const int contiguous = [] {
+      const char *e = std::getenv("QM_CONTIGUOUS");
+      return e ? std::atoi(e) : -1;
+    }();
+    if (contiguous >= 0 && contiguous < shape.K) shape.contiguous = contiguous;
+    static const int remap = [] {
+      const char *e = std::getenv("QM_REMAP");  // tuning knob: remapping mode 0..2
+      return e ? std::atoi(e) : -1;
+    }();
+    if (remap >= 0 && remap <= 2) shape.remap = remap;
   }
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
@@ -771,8 +792,7 @@ int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, in
     if (!fused_shape_supported(precision, lane_bits, R)) fail(QM_ERR_UNSUPPORTED, "tile shape is not compiled");
     if (qubits < tile_bits || qubits > 34) fail(QM_ERR_RANGE, "qubits");
     std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
-    FusedPlan plan = plan_fused(gates, qubits,
-                                FusedShape{tile_bits, lane_bits, R, precision == QM_C128 ? 3 : 4});
+    FusedPlan plan = plan_fused(gates, qubits, compiled_shape(tile_bits, lane_bits));
     *passes = jit_compile_check(plan, precision);
   });
 }
@@ -803,7 +823,7 @@ int qm_ctx_set_fused_shape(qm_ctx *h, int32_t precision, int32_t tile_bits, int3
         !(lane_bits >= 6 && lane_bits <= 10 && R >= 2 && R <= 5 && tile_bits <= max_k))
       fail(QM_ERR_UNSUPPORTED, "fused tile shape is not supported");
     std::lock_guard<std::recursive_mutex> g(c->lock);
-    c->shape[precision] = FusedShape{tile_bits, lane_bits, R, precision == QM_C128 ? 3 : 4};
+    c->shape[precision] = compiled_shape(tile_bits, lane_bits);
     for (auto &per_schedule : c->tuned)  // an explicit shape replaces tuned ones
       for (FusedShape &t : per_schedule[precision]) t = FusedShape{};
   });
@@ -848,7 +868,8 @@ int qm_autotune(qm_ctx *h, int32_t qubits, int32_t precision, int32_t schedule, 
         const int K = p[0], L = p[1], R = K - L;
         if (K > qubits || R < 2 || R > 5 || (precision == QM_C128 && K > 13) || K < 12 - (precision == QM_C128))
           continue;
-        cands.push_back(FusedShape{K, L, R, contiguous});
+        cands.push_back(compiled_shape(K, L));
+        if (K == 13 && L == 8) cands.push_back(FusedShape{K, L, R, contiguous});  // without remapping
       }
     }
     if (cands.empty()) fail(QM_ERR_UNSUPPORTED, "no tile shape fits this width");
@@ -876,6 +897,8 @@ int qm_autotune(qm_ctx *h, int32_t qubits, int32_t precision, int32_t schedule, 
       e.tile_bits = f.K;
       e.lane_bits = f.L;
       e.variant = f.variant;
+      e.contiguous = f.contiguous;
+      e.remap = f.remap;
       e.passes = st.passes;
       e.ms = ms;
       e.ms_per_gate = ms / (double)gates.size();
@@ -911,12 +934,14 @@ int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n
 }
 
 int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
-                       int32_t lane_bits, int32_t contiguous, double *amps) {
+                       int32_t lane_bits, int32_t contiguous, int32_t remap, double *amps) {
   return guarded([&] {
     if (qubits < 1 || qubits > 26) fail(QM_ERR_RANGE, "host reference is limited to 26 qubits");
     if (!amps) fail(QM_ERR_ARG, "amplitudes are null");
     std::vector<DevGate> g = prepare_all(ops, n_ops, qubits);
     FusedShape shape{tile_bits, lane_bits, tile_bits - lane_bits, contiguous};
+    shape.remap = remap;
+    if (remap < 0 || remap > 2) fail(QM_ERR_RANGE, "remap must be 0, 1 or 2");
     if (shape.R < 1 || shape.R > 5 || shape.L < 1 || shape.L > 10 || shape.K > kMaxK ||
         contiguous < 0 || contiguous >= shape.K)
       fail(QM_ERR_RANGE, "unsupported tile shape");

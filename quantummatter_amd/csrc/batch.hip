@@ -190,7 +190,7 @@ __device__ __forceinline__ void batch_pass(T *__restrict__ s, const DevPass &d,
           m0 += p;
       }
     }
-    if (!store_direct) lds_transpose<T, F, L, REGS>(lds, d, cur, kIo, t, vr, vi);
+    if (!store_direct) lds_transpose<T, F, L, REGS>(lds, d, cur, kIoStore, t, vr, vi);
     T *dst = s + (base | store_lane);
 #pragma unroll
     for (int r = 0; r < REGS; r++) {

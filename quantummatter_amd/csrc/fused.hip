@@ -122,9 +122,9 @@ __global__ __launch_bounds__(1 << L, (MODE & 1) ? 3 : 2) void k_fused(T *__restr
     }
     if (!store_direct) {
       if constexpr (SPLIT)
-        lds_transpose_split<F, L, REGS>(ldsf, d, cur, kIo, t, vr, vi);
+        lds_transpose_split<F, L, REGS>(ldsf, d, cur, kIoStore, t, vr, vi);
       else
-        lds_transpose<T, F, L, REGS>(lds, d, cur, kIo, t, vr, vi);
+        lds_transpose<T, F, L, REGS>(lds, d, cur, kIoStore, t, vr, vi);
     }
     T *dst = s + (base | store_lane);
 #pragma unroll

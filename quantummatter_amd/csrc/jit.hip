@@ -163,7 +163,8 @@ std::string hex(uint64_t v) {
 // Code-generation variant (tuning knob QM_JIT_VARIANT, default 1): bit 0
 // exchanges through an LDS buffer of reals (half the LDS, real then imaginary
 // parts, so more workgroups fit a CU), bit 2 forms the lane part of each LDS
-// address at its transpose instead of once per kernel.
+// address at its transpose instead of once per kernel, bit 3 loads and stores
+// the state with non-temporal hints (a pass never re-reads what it wrote).
 int jit_variant() {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
@@ -183,6 +184,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
                         std::vector<double> &coef) {
   const int variant = variant_of(sh);
   const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
+  const bool nontemporal = (variant & 8) != 0;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -217,7 +219,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     if (cur != ph) use(cur), use(ph);
     cur = ph;
   }
-  if (!d.store_direct) use(cur), use(kIo);
+  if (!d.store_direct) use(cur), use(kIoStore);
   auto lane_addr = [&](int slot) {
     std::string e = "(0u";
     for (int k = 0; k < L; k++)
@@ -234,9 +236,14 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   o << ";\n";
   o << "    F vr[" << N << "], vi[" << N << "];\n";
   o << "    { const " << T << " *src = s + (base | ll);\n";
-  for (int r = 0; r < N; r++)
-    o << "      { const " << T << " a = src[" << hex(d.load_reg_off[r]) << "]; vr[" << r << "] = a.re; vi["
-      << r << "] = a.im; }\n";
+  for (int r = 0; r < N; r++) {
+    if (nontemporal)
+      o << "      { const " << T << " *a = src + " << hex(d.load_reg_off[r]) << "; vr[" << r
+        << "] = __builtin_nontemporal_load(&a->re); vi[" << r << "] = __builtin_nontemporal_load(&a->im); }\n";
+    else
+      o << "      { const " << T << " a = src[" << hex(d.load_reg_off[r]) << "]; vr[" << r << "] = a.re; vi["
+        << r << "] = a.im; }\n";
+  }
   o << "    }\n";
   auto transpose = [&](int from, int to) {
     o << "    {\n";
@@ -368,11 +375,16 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       o << "      qj_phase" << tmpl << "0>(vr, vi, pr, pi); }\n";
     }
   }
-  if (!d.store_direct) transpose(cur, kIo);
+  if (!d.store_direct) transpose(cur, kIoStore);
   o << "    { " << T << " *dst = s + (base | sl);\n";
-  for (int r = 0; r < N; r++)
-    o << "      { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; dst[" << hex(d.store_reg_off[r])
-      << "] = a; }\n";
+  for (int r = 0; r < N; r++) {
+    if (nontemporal)
+      o << "      { " << T << " *a = dst + " << hex(d.store_reg_off[r]) << "; __builtin_nontemporal_store(vr[" << r
+        << "], &a->re); __builtin_nontemporal_store(vi[" << r << "], &a->im); }\n";
+    else
+      o << "      { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; dst["
+        << hex(d.store_reg_off[r]) << "] = a; }\n";
+  }
   o << "    }\n  }\n}\n";
   return o.str();
 }

@@ -238,12 +238,19 @@ void merge_phase_diagonals(std::vector<TileOp> &ops, const std::vector<uint64_t>
 // Rank of bit p among the set bits of m (number of set bits below p).
 inline int rank_in(uint32_t m, int p) { return popc(m & ((1u << p) - 1)); }
 
-void fill_mapping(DevPass &d, int slot, uint32_t rmask, int K, int R) {
-  int r = 0, k = 0;
+// Register set rmask, thread bits in ascending local position after the
+// `nfirst` positions `first` (which must not be register positions).
+void fill_mapping(DevPass &d, int slot, uint32_t rmask, int K, int R, const int *first = nullptr,
+                  int nfirst = 0) {
+  int k = 0;
+  uint32_t placed = 0;
+  for (int i = 0; i < nfirst; i++) {
+    d.lane_swz[slot][k] = lds_swizzle(1u << first[i]);
+    d.lanepos[slot][k++] = (uint8_t)first[i];
+    placed |= 1u << first[i];
+  }
   for (int p = 0; p < K; p++) {
-    if (rmask & (1u << p)) {
-      (void)r;
-    } else {
+    if (!(rmask & (1u << p)) && !(placed & (1u << p))) {
       d.lane_swz[slot][k] = lds_swizzle(1u << p);
       d.lanepos[slot][k++] = (uint8_t)p;
     }
@@ -279,18 +286,32 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
   plan.source_gates = (int64_t)gates_in.size();
   if (n < sh.K) fail(QM_ERR_ARG, "fused passes need at least K qubits");
   const int K = sh.K, L = sh.L, R = sh.R, c = sh.contiguous;
+  const bool remap = sh.remap != 0 && c > 0 && 2 * c <= K;
   const uint64_t all = n >= 64 ? ~0ull : (((uint64_t)1 << n) - 1);
-  const uint64_t low = ((uint64_t)1 << c) - 1;
   const uint32_t low_local = (1u << c) - 1;
+  // Assignment of qubits to index bits: logi[x] is the qubit at index bit x.
+  // Invariant: for p < c, logi[p] == p, or logi[p] = h >= c and logi[h] = p
+  // (a transposition); every other bit holds its own qubit.
+  int logi[64];
+  for (int x = 0; x < 64; x++) logi[x] = x;
+  auto displaced = [&] {
+    for (int p = 0; p < c; p++)
+      if (logi[p] != p) return true;
+    return false;
+  };
 
   std::vector<PGate> G = expand(gates_in);
   std::vector<int> pending(G.size());
   for (size_t i = 0; i < G.size(); i++) pending[i] = (int)i;
 
-  while (!pending.empty()) {
+  // A last pass with no gates restores the natural order if the final gate
+  // pass could not.
+  while (!pending.empty() || displaced()) {
     // 1. Tile qubits: greedy scan with dependency blocking; spare tile slots
     //    go to the targets of the deferred gates, then the scan is repeated.
-    uint64_t Q = low;
+    //    The qubits at the low index bits are always in the tile.
+    uint64_t Q = 0;
+    for (int p = 0; p < c; p++) Q |= (uint64_t)1 << logi[p];
     std::vector<int> included, deferred;
     auto scan_tile = [&](uint64_t &q, bool grow) {
       uint64_t blocked = 0;
@@ -321,31 +342,46 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         included.push_back(idx);
       }
     };
+    // Spare slots: qubits displaced from the low bits first (so their
+    // transpositions can be undone), then the lowest qubits.
+    auto fill = [&](uint64_t &q) {
+      for (int p = 0; p < c && popc(q) < K; p++)
+        if (logi[p] != p) q |= (uint64_t)1 << p;
+      for (int x = 0; popc(q) < K && x < n; x++) q |= (uint64_t)1 << x;
+    };
     scan_tile(Q, true);
     if (popc(Q) < K && !deferred.empty()) {
       for (int idx : deferred) {
         if (popc(Q) >= K) break;
         if (G[idx].pair) Q |= (uint64_t)1 << G[idx].g.target;
       }
-      for (int q = 0; popc(Q) < K && q < n; q++) Q |= (uint64_t)1 << q;
+      fill(Q);
       scan_tile(Q, false);
     }
-    for (int q = 0; popc(Q) < K && q < n; q++) Q |= (uint64_t)1 << q;
+    fill(Q);
+
+    int phys[64];
+    for (int x = 0; x < 64; x++) phys[logi[x]] = x;
+    uint64_t P = 0;  // index bits of the tile
+    for (int q = 0; q < n; q++)
+      if (Q & ((uint64_t)1 << q)) P |= (uint64_t)1 << phys[q];
 
     FusedPass pass;
     DevPass &d = pass.dev;
     std::memset(&d, 0, sizeof(d));
-    d.qmask = Q;
+    d.qmask = P;
     d.n = n;
+    // Local positions follow the index bits in ascending order; local_of is
+    // indexed by qubit.
     int qpos[64], local_of[64];
     for (int q = 0; q < 64; q++) local_of[q] = -1;
     {
       int k = 0;
-      for (int q = 0; q < n; q++)
-        if (Q & ((uint64_t)1 << q)) {
-          qpos[k] = q;
-          local_of[q] = k;
-          d.qpos[k] = (uint8_t)q;
+      for (int x = 0; x < n; x++)
+        if (P & ((uint64_t)1 << x)) {
+          qpos[k] = x;
+          local_of[logi[x]] = k;
+          d.qpos[k] = (uint8_t)x;
           k++;
         }
     }
@@ -353,12 +389,12 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     {
       int src = 0, nr = 0;
       for (int q = 0; q < n;) {
-        if (Q & ((uint64_t)1 << q)) {
+        if (P & ((uint64_t)1 << q)) {
           q++;
           continue;
         }
         int start = q;
-        while (q < n && !(Q & ((uint64_t)1 << q))) q++;
+        while (q < n && !(P & ((uint64_t)1 << q))) q++;
         d.run_src[nr] = (uint8_t)src;
         d.run_dst[nr] = (uint8_t)start;
         d.run_len[nr] = (uint8_t)(q - start);
@@ -429,31 +465,182 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
       deferred.insert(deferred.end(), overflow.begin(), overflow.end());
       std::sort(deferred.begin(), deferred.end());
     }
-
-    // 3. Mappings and ops.
     uint32_t io_rmask = 0;
     for (int p = K - 1; popc(io_rmask) < R; p--) io_rmask |= 1u << p;
+    if (rsets.empty()) {  // a pass without gates (restores the natural order)
+      rsets.push_back(io_rmask);
+      phase_gates.emplace_back();
+    }
+
+    // 3. Remapping: the tile's qubits whose next use as a pair target comes
+    //    first among the remaining gates are stored at the low index bits.
+    //    They are taken among the thread bits of the last phase, whose thread
+    //    order is then chosen so the store needs no transpose: thread bit i
+    //    holds the qubit stored to index bit i.  When the last phase is also
+    //    the load phase its thread order is fixed and nothing moves.
+    const int last = (int)rsets.size() - 1;
+    const bool fixed_lanes = last == 0 && (rsets[0] & low_local) == 0;
+    auto on_lane = [&](int q) { return (rsets[last] & (1u << local_of[q])) == 0; };
+    int newlogi[64];
+    for (int x = 0; x < 64; x++) newlogi[x] = logi[x];
+    bool lane_order = false;  // last phase reordered for a direct store
+    auto undoable = [&](int p) { return newlogi[p] == p || (P & ((uint64_t)1 << newlogi[p])); };
+    if (remap && deferred.empty()) {
+      // No gates left: restore the natural order as far as the tile allows.
+      lane_order = !fixed_lanes;
+      for (int p = 0; p < c; p++) {
+        if (newlogi[p] != p && undoable(p)) {
+          const int h = newlogi[p];
+          newlogi[p] = p;
+          newlogi[h] = h;
+        }
+        lane_order = lane_order && on_lane(newlogi[p]);
+      }
+    } else if (remap && !fixed_lanes) {
+      int first_use[64];
+      for (int q = 0; q < 64; q++) first_use[q] = INT32_MAX;
+      for (size_t k = 0; k < deferred.size(); k++) {
+        const PGate &x = G[deferred[k]];
+        if (x.pair && first_use[x.g.target] == INT32_MAX) first_use[x.g.target] = (int)k;
+      }
+      std::vector<std::pair<int, int>> cand, spare;
+      for (int q = 0; q < n; q++)
+        if ((Q & ((uint64_t)1 << q)) && on_lane(q))
+          (first_use[q] != INT32_MAX ? cand : spare).push_back({first_use[q], q});
+      std::sort(cand.begin(), cand.end());
+      if ((int)cand.size() > c) {
+        spare.insert(spare.begin(), cand.begin() + c, cand.end());
+        cand.resize((size_t)c);
+      }
+      bool reserved[64] = {};
+      auto undo = [&](int p) {
+        const int h = newlogi[p];
+        newlogi[p] = p;
+        newlogi[h] = h;
+      };
+      // Low qubits wanted low go home (their partner bit is in the tile).
+      for (const auto &e : cand)
+        if (e.second < c) {
+          if (newlogi[e.second] != e.second) undo(e.second);
+          reserved[e.second] = true;
+        }
+      // Wanted qubits already at a low bit stay there.
+      std::vector<int> want;
+      for (const auto &e : cand) {
+        const int b = e.second;
+        if (b < c) continue;
+        int at = -1;
+        for (int p = 0; p < c; p++)
+          if (newlogi[p] == b) at = p;
+        if (at >= 0)
+          reserved[at] = true;
+        else
+          want.push_back(b);
+      }
+      // The others (at their own bit b) take a free low bit p, preferring one
+      // whose qubit is a register of the last phase: qubit b to p, qubit p to
+      // bit b (p's transposition undone first when its partner is in the tile).
+      auto place = [&](int b, int p) {
+        if (newlogi[p] != p) undo(p);
+        newlogi[p] = b;
+        newlogi[b] = p;
+        reserved[p] = true;
+      };
+      for (int b : want) {
+        int p = -1;
+        for (int x = 0; x < c && p < 0; x++)
+          if (!reserved[x] && !on_lane(newlogi[x]) && undoable(x)) p = x;
+        for (int x = 0; x < c && p < 0; x++)
+          if (!reserved[x] && newlogi[x] == x) p = x;
+        for (int x = 0; x < c && p < 0; x++)
+          if (!reserved[x] && undoable(x)) p = x;
+        if (p >= 0) place(b, p);
+      }
+      // Other transpositions are undone where the tile holds both bits and the
+      // returning qubit is a thread bit of the last phase.
+      for (int p = 0; p < c; p++)
+        if (!reserved[p] && newlogi[p] != p && undoable(p) && on_lane(p)) undo(p);
+      // Low bits still holding a register qubit of the last phase take a spare
+      // thread-bit qubit (earliest next use first).
+      lane_order = true;
+      size_t next_spare = 0;
+      for (int p = 0; p < c && lane_order; p++) {
+        if (on_lane(newlogi[p])) continue;
+        int b = -1;
+        while (next_spare < spare.size() && b < 0) {
+          const int q = spare[next_spare++].second;
+          if (q >= c && newlogi[q] == q) b = q;
+        }
+        if (b < 0 || !undoable(p)) {
+          lane_order = false;
+          break;
+        }
+        place(b, p);
+      }
+    }
+    if (sh.remap == 2 && !lane_order && !deferred.empty()) {
+      // Moves that would need a transpose before the store are dropped.
+      for (int x = 0; x < 64; x++) newlogi[x] = logi[x];
+    }
+    int newphys[64];
+    for (int x = 0; x < 64; x++) newphys[newlogi[x]] = x;
+    int store_pos[16];  // index bit each local position is stored to
+    bool moved = false;
+    for (int k = 0; k < K; k++) {
+      store_pos[k] = newphys[logi[qpos[k]]];
+      moved |= store_pos[k] != qpos[k];
+    }
+
+    // 4. Mappings and ops.
     d.nphases = (int32_t)rsets.size();
     for (int ph = 0; ph < d.nphases; ph++) fill_mapping(d, ph, rsets[ph], K, R);
     fill_mapping(d, kIo, io_rmask, K, R);
     d.load_direct = (rsets.front() & low_local) == 0;
-    d.store_direct = (rsets.back() & low_local) == 0;
+    fill_mapping(d, kIoStore, io_rmask, K, R);
+    if (!moved) {
+      d.store_direct = (rsets.back() & low_local) == 0;
+    } else {
+      // Thread bits 0..c-1 must hold the local positions stored to index bits
+      // 0..c-1, in that order, for coalesced stores.
+      int low_src[16];
+      uint32_t low_set = 0;
+      for (int k = 0; k < K; k++)
+        if (store_pos[k] < c) {
+          low_src[store_pos[k]] = k;
+          low_set |= 1u << k;
+        }
+      if (lane_order) {
+        fill_mapping(d, last, rsets[last], K, R, low_src, c);
+        d.store_direct = 1;
+      } else {
+        uint32_t st_rmask = 0;
+        for (int p = K - 1; popc(st_rmask) < R; p--)
+          if (!(low_set & (1u << p))) st_rmask |= 1u << p;
+        fill_mapping(d, kIoStore, st_rmask, K, R, low_src, c);
+        d.store_direct = 0;
+      }
+    }
+    // Thread bit of each local position per phase mapping.
+    int lane_of[kSlots][16];
+    for (int ph = 0; ph < d.nphases; ph++)
+      for (int k = 0; k < L; k++) lane_of[ph][d.lanepos[ph][k]] = k;
     pass.transposes = (d.nphases - 1) + (d.load_direct ? 0 : 1) + (d.store_direct ? 0 : 1);
-    auto set_io = [&](int slot, uint64_t *reg_off, uint8_t *lane_gbit, uint64_t *lane_off) {
+    auto set_io = [&](int slot, const int *pos, uint64_t *reg_off, uint8_t *lane_gbit,
+                      uint64_t *lane_off) {
       for (int reg = 0; reg < (1 << R); reg++) {
         uint32_t j = d.regloc[slot][reg];
         uint64_t g = 0;
         for (int p = 0; p < K; p++)
-          if (j & (1u << p)) g |= (uint64_t)1 << qpos[p];
+          if (j & (1u << p)) g |= (uint64_t)1 << pos[p];
         reg_off[reg] = g;
       }
       for (int k = 0; k < L; k++) {
-        lane_gbit[k] = (uint8_t)qpos[d.lanepos[slot][k]];
+        lane_gbit[k] = (uint8_t)pos[d.lanepos[slot][k]];
         lane_off[k] = (uint64_t)1 << lane_gbit[k];
       }
     };
-    set_io(d.load_direct ? 0 : kIo, d.load_reg_off, d.load_lane_gbit, d.load_lane_off);
-    set_io(d.store_direct ? d.nphases - 1 : kIo, d.store_reg_off, d.store_lane_gbit,
+    set_io(d.load_direct ? 0 : kIo, qpos, d.load_reg_off, d.load_lane_gbit, d.load_lane_off);
+    set_io(d.store_direct ? d.nphases - 1 : kIoStore, store_pos, d.store_reg_off, d.store_lane_gbit,
            d.store_lane_off);
 
     std::vector<uint64_t> op_touch, op_pair_target;
@@ -475,21 +662,21 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           if (!(x.g.ctrl_mask & ((uint64_t)1 << q))) continue;
           int lp = local_of[q];
           if (lp < 0)
-            t.out_cmask |= (uint64_t)1 << q;
+            t.out_cmask |= (uint64_t)1 << phys[q];
           else if (rset & (1u << lp))
             t.reg_cmask |= 1u << rank_in(rset, lp);
           else
-            t.lane_cmask |= 1u << rank_in(~rset & ((1u << K) - 1), lp);
+            t.lane_cmask |= 1u << lane_of[ph][lp];
         }
         int lp = local_of[x.g.target];
         if (x.pair) {
           t.rbit = rank_in(rset, lp);
         } else if (lp < 0) {
-          t.dout = (uint64_t)1 << x.g.target;
+          t.dout = (uint64_t)1 << phys[x.g.target];
         } else if (rset & (1u << lp)) {
           t.dreg = 1u << rank_in(rset, lp);
         } else {
-          t.dlane = 1u << rank_in(~rset & ((1u << K) - 1), lp);
+          t.dlane = 1u << lane_of[ph][lp];
         }
         if (t.rbit >= 0 && t.reg_cmask != 0 && t.kind != G_X && t.kind < G_PHASEPROD) {
           // Register-controlled: the kernel runs the general 2x2 form, selecting the
@@ -520,6 +707,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     pass.rsets = rsets;
     plan.passes.push_back(std::move(pass));
     pending.swap(deferred);
+    for (int x = 0; x < 64; x++) logi[x] = newlogi[x];
   }
   return plan;
 }
@@ -848,7 +1036,7 @@ void apply_plan_host(const FusedPlan &plan, double *s, int n) {
           }
         }
       }
-      if (!d.store_direct) transpose(cur, kIo);
+      if (!d.store_direct) transpose(cur, kIoStore);
       for (int t = 0; t < threads; t++) {
         uint64_t lo = 0;
         for (int k = 0; k < L; k++)

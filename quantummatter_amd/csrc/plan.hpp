@@ -71,11 +71,21 @@ double touched_fraction(const DevGate &g);
 // bits changes between phases by an LDS transpose.  A pair gate's target must
 // be a register bit of its phase; a diagonal gate's target and every control
 // may be a register bit, a lane bit or a qubit outside the tile.
+//
+// Remapping (FusedShape::remap): a pass may store its tile under a different
+// assignment of qubits to its index bits than it loaded it with, so the low
+// `contiguous` index bits (the contiguous runs) hold qubits the next pass
+// needs instead of always qubits 0..contiguous-1.  The assignment is kept as
+// disjoint transpositions of a low index bit p with a higher one h (qubit h at
+// index bit p, qubit p at index bit h), and every plan ends with the identity
+// assignment, so states outside a plan are always in natural order.
 // ---------------------------------------------------------------------------
 constexpr int kMaxK = 14;
 constexpr int kMaxPhases = 12;
 constexpr int kMaxRegs = 32;
-constexpr int kIo = kMaxPhases;  // mapping slot of the I/O layout
+constexpr int kIo = kMaxPhases;             // mapping slot of the load layout
+constexpr int kIoStore = kMaxPhases + 1;     // mapping slot of the store layout
+constexpr int kSlots = kMaxPhases + 2;
 
 struct TileOp {
   int32_t kind;         // GKind (swaps are planned as three exchanges)
@@ -115,18 +125,18 @@ struct DevPass {
   uint32_t mass_dlane;
   uint32_t mass_pad;
   uint64_t mass_dout;
-  uint16_t regloc[kMaxPhases + 1][kMaxRegs];  // local index of register r per mapping
-  uint16_t regswz[kMaxPhases + 1][kMaxRegs];  // lds_swizzle(regloc), see lds_swizzle()
-  uint32_t lane_swz[kMaxPhases + 1][16];      // lds_swizzle(1 << lanepos) of thread bit k
+  uint16_t regloc[kSlots][kMaxRegs];  // local index of register r per mapping
+  uint16_t regswz[kSlots][kMaxRegs];  // lds_swizzle(regloc), see lds_swizzle()
+  uint32_t lane_swz[kSlots][16];      // lds_swizzle(1 << lanepos) of thread bit k
   uint64_t load_lane_off[16];                 // global offset of thread bit k (load mapping)
   uint64_t store_lane_off[16];                // ... (store mapping)
-  uint8_t lanepos[kMaxPhases + 1][16];        // local position of thread bit k per mapping
+  uint8_t lanepos[kSlots][16];        // local position of thread bit k per mapping
   uint8_t load_lane_gbit[16];                 // global bit of thread bit k under the load mapping
   uint8_t store_lane_gbit[16];
   uint8_t run_src[16];
   uint8_t run_dst[16];
   uint8_t run_len[16];
-  uint8_t qpos[16];  // tile qubits ascending
+  uint8_t qpos[16];  // tile index bits (physical), ascending
 };
 
 // LDS address swizzle: XOR of the low 3 bits of a 16-byte-slot index with a
@@ -168,8 +178,13 @@ struct FusedShape {
   int K;           // tile bits
   int L;           // lane bits (threads = 2^L)
   int R;           // register bits
-  int contiguous;  // lowest qubits always in the tile
+  int contiguous;  // lowest index bits always in the tile
   int variant = -1;  // code variant: interpreter mode or generator variant (-1: default)
+  // 0: the lowest `contiguous` qubits stay at the low index bits; 1: passes
+  // move qubits into and out of them (a transpose before the store where the
+  // last phase cannot store directly); 2: as 1, but only moves that keep the
+  // store direct (and the final restore).
+  int remap = 0;
 };
 
 struct FusedPass {

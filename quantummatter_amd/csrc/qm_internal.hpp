@@ -54,6 +54,14 @@ int guarded(Fn &&fn) {
   }
 }
 
+// Default form of a compiled-schedule shape: 2^K-amplitude tiles over 2^L
+// threads, the lowest 4 index bits in every tile, remapped between passes.
+inline FusedShape compiled_shape(int K, int L) {
+  FusedShape s{K, L, K - L, 4};
+  s.remap = 1;
+  return s;
+}
+
 struct Ctx {
   int device = 0;
   bool profiling = false;
@@ -80,9 +88,9 @@ struct Ctx {
   // Tile shape per precision (qm_ctx_set_fused_shape).  The compiled schedule
   // uses it as given; the interpreting kernel uses it when it is one of its
   // instantiations and its own default otherwise.  Defaults: 8192-amplitude
-  // tiles, 256 threads x 32 registers (the compiled schedule's best on the
-  // 30-qubit layer workload).
-  FusedShape shape[2] = {FusedShape{13, 8, 5, 3}, FusedShape{13, 8, 5, 4}};
+  // tiles, 256 threads x 32 registers, the low 4 index bits remapped between
+  // passes (the compiled schedule's best on the 30-qubit layer workload).
+  FusedShape shape[2] = {compiled_shape(13, 8), compiled_shape(13, 8)};
   // Shapes chosen by qm_autotune, [fused 0 / compiled 1][precision][qubits];
   // K == 0: not tuned.  They take precedence over `shape` for their width.
   FusedShape tuned[2][2][35] = {};
