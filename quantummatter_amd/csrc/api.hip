@@ -789,7 +789,9 @@ int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, in
     if (!passes) fail(QM_ERR_ARG, "passes is null");
     if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
     const int R = tile_bits - lane_bits;
-    if (!fused_shape_supported(precision, lane_bits, R)) fail(QM_ERR_UNSUPPORTED, "tile shape is not compiled");
+    const int max_k = precision == QM_C128 ? 13 : 14;
+    if (!(lane_bits >= 6 && lane_bits <= 10 && R >= 2 && R <= 5 && tile_bits <= max_k))
+      fail(QM_ERR_UNSUPPORTED, "the compiled schedule does not generate this tile shape");
     if (qubits < tile_bits || qubits > 34) fail(QM_ERR_RANGE, "qubits");
     std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
     FusedPlan plan = plan_fused(gates, qubits, compiled_shape(tile_bits, lane_bits));

@@ -14,6 +14,7 @@
 #include <unistd.h>
 
 #include <cerrno>
+#include <cmath>
 
 #include <algorithm>
 #include <atomic>
@@ -125,6 +126,62 @@ __device__ __forceinline__ void qj_mat(F (&vr)[N], F (&vi)[N], const double *c) 
     vi[q] = qj_fma(c6, y1, qj_fma(c7, y0, qj_fma(c4, x1, c5 * x0)));
   }
 }
+// Uncontrolled rotations with a common factor taken out (applied once per
+// pass): Rx = c (I - i t X) with t = s/c (form a, |c| >= |s|) or s (u I - i X)
+// with u = c/s (form b); Ry likewise with -i X replaced by the real -iY.
+// One fused multiply-add per output.
+template <typename F, int N, int B>
+__device__ __forceinline__ void qj_rxa(F (&vr)[N], F (&vi)[N], F t) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if (r & (1 << B)) continue;
+    const int q = r | (1 << B);
+    const F xr = vr[r], xi = vi[r], yr = vr[q], yi = vi[q];
+    vr[r] = qj_fma(t, yi, xr);
+    vi[r] = qj_fma(-t, yr, xi);
+    vr[q] = qj_fma(t, xi, yr);
+    vi[q] = qj_fma(-t, xr, yi);
+  }
+}
+template <typename F, int N, int B>
+__device__ __forceinline__ void qj_rxb(F (&vr)[N], F (&vi)[N], F u) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if (r & (1 << B)) continue;
+    const int q = r | (1 << B);
+    const F xr = vr[r], xi = vi[r], yr = vr[q], yi = vi[q];
+    vr[r] = qj_fma(u, xr, yi);
+    vi[r] = qj_fma(u, xi, -yr);
+    vr[q] = qj_fma(u, yr, xi);
+    vi[q] = qj_fma(u, yi, -xr);
+  }
+}
+template <typename F, int N, int B>
+__device__ __forceinline__ void qj_rya(F (&vr)[N], F (&vi)[N], F t) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if (r & (1 << B)) continue;
+    const int q = r | (1 << B);
+    const F xr = vr[r], xi = vi[r], yr = vr[q], yi = vi[q];
+    vr[r] = qj_fma(-t, yr, xr);
+    vi[r] = qj_fma(-t, yi, xi);
+    vr[q] = qj_fma(t, xr, yr);
+    vi[q] = qj_fma(t, xi, yi);
+  }
+}
+template <typename F, int N, int B>
+__device__ __forceinline__ void qj_ryb(F (&vr)[N], F (&vi)[N], F u) {
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if (r & (1 << B)) continue;
+    const int q = r | (1 << B);
+    const F xr = vr[r], xi = vi[r], yr = vr[q], yi = vi[q];
+    vr[r] = qj_fma(u, xr, -yr);
+    vi[r] = qj_fma(u, xi, -yi);
+    vr[q] = qj_fma(u, yr, xr);
+    vi[q] = qj_fma(u, yi, xi);
+  }
+}
 template <typename F>
 __device__ __forceinline__ void qj_mul(F &xr, F &xi, F pr, F pi) {
   const F a = xi * pi, b = xr * pi;
@@ -160,15 +217,19 @@ std::string hex(uint64_t v) {
 }
 
 // Source of one pass kernel, and the coefficient layout it reads.
-// Code-generation variant (tuning knob QM_JIT_VARIANT, default 1): bit 0
+// Code-generation variant (tuning knob QM_JIT_VARIANT, default 17): bit 0
 // exchanges through an LDS buffer of reals (half the LDS, real then imaginary
 // parts, so more workgroups fit a CU), bit 2 forms the lane part of each LDS
 // address at its transpose instead of once per kernel, bit 3 loads and stores
-// the state with non-temporal hints (a pass never re-reads what it wrote).
+// the state with non-temporal hints (a pass never re-reads what it wrote),
+// bit 4 takes the common factor out of uncontrolled rotations and register
+// diagonals (ratio forms, one multiply-add per output) and applies the
+// product of the factors once per pass, bit 5 keeps exchanges conditioned on
+// thread or outside bits as per-tile flags (X frames, see pass_source).
 int jit_variant() {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
-    return e ? std::atoi(e) : 1;
+    return e ? std::atoi(e) : 17;
   }();
   return v;
 }
@@ -184,7 +245,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
                         std::vector<double> &coef) {
   const int variant = variant_of(sh);
   const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
-  const bool nontemporal = (variant & 8) != 0;
+  const bool nontemporal = (variant & 8) != 0, factored = (variant & 16) != 0, frames = (variant & 32) != 0;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -272,15 +333,6 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     }
     o << "    }\n";
   };
-  auto cond = [&](const TileOp &op) {
-    std::string c;
-    if (op.lane_cmask) c += "(t & " + std::to_string(op.lane_cmask) + "u) == " + std::to_string(op.lane_cmask) + "u";
-    if (op.out_cmask) {
-      if (!c.empty()) c += " && ";
-      c += "(base & " + hex(op.out_cmask) + ") == " + hex(op.out_cmask);
-    }
-    return c;
-  };
   auto cfix = [&](const double *v, int n) {
     const size_t at = coef.size();
     coef.insert(coef.end(), v, v + n);
@@ -289,30 +341,198 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const std::string tmpl = std::string("<F, ") + std::to_string(N) + ", ";
   cur = d.load_direct ? 0 : kIo;
   const TileOp *ops = p.ops.data();
+  const int K = sh.K;
+  auto lp_reg = [&](int slot, int bit) { return __builtin_ctz((unsigned)d.regloc[slot][1u << bit]); };
+  auto fname = [](int lp) { return "fq" + std::to_string(lp); };
+  // X frames: an exchange on register bit b conditioned on lane or outside bits
+  // only is not performed on the registers; it sets a per-tile flag fq<lp> for
+  // its tile position.  While the flag is set, the data of that position is
+  // held with the bit flipped: conditions on it are inverted, Ry and diagonal
+  // coefficients are conjugated by X, and the store address flips the bit.  A
+  // register control, or an H, Y or general matrix on it, performs the pending
+  // exchange first.  A flag set under thread-bit conditions describes the data
+  // each thread holds, so it stays valid across an LDS transpose only while its
+  // condition positions keep their thread bits and its position stays a
+  // register bit; otherwise the exchange is performed before the transpose.
+  struct Frame {
+    bool lane = false;                     // depends on thread bits
+    std::vector<std::pair<int, int>> deps;  // (tile position, thread bit)
+  };
+  std::vector<char> framed((size_t)K, 0);
+  std::vector<Frame> fdep((size_t)K);
+  auto is_source = [&](const TileOp &op) {
+    return frames && op.kind == G_X && op.rbit >= 0 && op.reg_cmask == 0 && (op.lane_cmask || op.out_cmask);
+  };
+  std::vector<char> declared((size_t)K, 0);
+  for (int ph = 0; ph < d.nphases; ph++)
+    for (int i = 0; i < d.phase_op_count[ph]; i++) {
+      const TileOp &op = ops[d.phase_op_begin[ph] + i];
+      if (is_source(op)) declared[(size_t)lp_reg(ph, op.rbit)] = 1;
+    }
+  for (int lp = 0; lp < K; lp++)
+    if (declared[(size_t)lp]) o << "    bool " << fname(lp) << " = false;\n";
+  auto cond = [&](const TileOp &op, int slot) {
+    std::vector<std::string> parts;
+    uint32_t plain = 0;
+    for (int k = 0; k < L; k++)
+      if (op.lane_cmask & (1u << k)) {
+        const int lp = d.lanepos[slot][k];
+        if (framed[(size_t)lp])
+          parts.push_back("((((t >> " + std::to_string(k) + ") & 1u) != 0u) != " + fname(lp) + ")");
+        else
+          plain |= 1u << k;
+      }
+    if (plain) parts.insert(parts.begin(), "(t & " + std::to_string(plain) + "u) == " + std::to_string(plain) + "u");
+    if (op.out_cmask) parts.push_back("(base & " + hex(op.out_cmask) + ") == " + hex(op.out_cmask));
+    std::string c;
+    for (const std::string &x : parts) c += (c.empty() ? "" : " && ") + x;
+    return c;
+  };
+  auto hi_expr = [&](const TileOp &op, int slot) {
+    std::string hi;
+    if (op.dlane) {
+      const int lp = d.lanepos[slot][__builtin_ctz(op.dlane)];
+      hi = "((t & " + std::to_string(op.dlane) + "u) != 0u)";
+      if (framed[(size_t)lp]) hi = "(" + hi + " != " + fname(lp) + ")";
+    }
+    if (op.dout) hi += std::string(hi.empty() ? "" : " || ") + "(base & " + hex(op.dout) + ") != 0ull";
+    return hi.empty() ? std::string("false") : hi;
+  };
+  // Performs the pending exchange of register bit `bit` of mapping `slot`.
+  auto settle = [&](int slot, int bit) {
+    const int lp = lp_reg(slot, bit);
+    if (!framed[(size_t)lp]) return;
+    o << "    if (" << fname(lp) << ") { qj_x" << tmpl << bit << ", 0>(vr, vi); } " << fname(lp) << " = false;\n";
+    framed[(size_t)lp] = 0;
+    fdep[(size_t)lp] = Frame{};
+  };
+  auto reg_bit = [&](int slot, int lp) {
+    for (int b = 0; b < R; b++)
+      if (lp_reg(slot, b) == lp) return b;
+    return -1;
+  };
+  // LDS transpose, first performing the exchanges whose flags would not
+  // survive it.
+  auto move_to = [&](int from, int to) {
+    for (int lp = 0; lp < K; lp++) {
+      if (!framed[(size_t)lp] || !fdep[(size_t)lp].lane) continue;
+      bool keep = reg_bit(to, lp) >= 0;
+      for (const auto &dk : fdep[(size_t)lp].deps) keep = keep && d.lanepos[to][dk.second] == dk.first;
+      if (!keep) settle(from, reg_bit(from, lp));
+    }
+    transpose(from, to);
+  };
+  auto sel = [&](int lp, const std::string &when_set, const std::string &otherwise) {
+    return framed[(size_t)lp] ? "(" + fname(lp) + " ? " + when_set + " : " + otherwise + ")" : otherwise;
+  };
+  // Factored forms: uncontrolled ops only (their factor is a scalar multiple of
+  // the whole state, so it commutes with everything and is applied at the end
+  // of the last phase).
+  auto uncontrolled = [](const TileOp &op) { return !op.reg_cmask && !op.lane_cmask && !op.out_cmask; };
+  auto factor_rot = [&](const TileOp &op) {
+    return factored && op.rbit >= 0 && (op.kind == G_RX || op.kind == G_RY) && uncontrolled(op);
+  };
+  auto factor_diag = [&](const TileOp &op) {
+    return factored && op.rbit < 0 && op.kind == G_DIAG && op.dreg && uncontrolled(op);
+  };
+  bool any_factor = false;
+  for (int ph = 0; ph < d.nphases; ph++)
+    for (int i = 0; i < d.phase_op_count[ph]; i++) {
+      const TileOp &op = ops[d.phase_op_begin[ph] + i];
+      any_factor |= factor_rot(op) || factor_diag(op);
+    }
+  double gre = 1, gim = 0;  // product of the factors taken out
+  size_t gslot = 0;
+  const bool use_gneg = any_factor && std::find(declared.begin(), declared.end(), 1) != declared.end();
+  if (any_factor) {
+    const double zero[2] = {0, 0};
+    gslot = cfix(zero, 2);
+  }
+  if (use_gneg) o << "    bool gneg = false;\n";
+  auto cf_str = [](size_t k) { return "(F)cf[" + std::to_string(k) + "]"; };
   for (int ph = 0; ph < d.nphases; ph++) {
-    if (cur != ph) transpose(cur, ph);
+    if (cur != ph) move_to(cur, ph);
     cur = ph;
     for (int i = 0; i < d.phase_op_count[ph]; i++) {
       const TileOp &op = ops[d.phase_op_begin[ph] + i];
-      const std::string c = cond(op);
+      for (int b = 0; b < R; b++)
+        if (op.reg_cmask & (1u << b)) settle(ph, b);
+      const int tlp = op.rbit >= 0 ? lp_reg(ph, op.rbit) : (op.dreg ? lp_reg(ph, __builtin_ctz(op.dreg)) : -1);
+      if (op.rbit >= 0 && (op.kind == G_H || op.kind == G_Y || op.kind == G_MAT)) settle(ph, op.rbit);
+      if (factor_rot(op) && op.kind == G_RY && framed[(size_t)tlp] && fdep[(size_t)tlp].lane &&
+          std::fabs(op.c[0]) < std::fabs(op.c[1]))
+        settle(ph, op.rbit);  // its factor's sign would depend on the thread's data
+      if (is_source(op)) {
+        o << "    " << fname(tlp) << " ^= (" << cond(op, ph) << ");\n";
+        Frame &f = fdep[(size_t)tlp];
+        for (int k = 0; k < L; k++)
+          if (op.lane_cmask & (1u << k)) {
+            const int c = d.lanepos[ph][k];
+            f.lane = true;
+            f.deps.push_back({c, k});
+            if (framed[(size_t)c]) {
+              f.lane = f.lane || fdep[(size_t)c].lane;
+              f.deps.insert(f.deps.end(), fdep[(size_t)c].deps.begin(), fdep[(size_t)c].deps.end());
+            }
+          }
+        framed[(size_t)tlp] = 1;
+        continue;
+      }
+      const std::string c = cond(op, ph);
       o << "    " << (c.empty() ? "{" : "if (" + c + ") {") << " ";
       const std::string m = std::to_string(op.reg_cmask);
-      if (op.rbit >= 0) {
+      if (factor_rot(op)) {
+        const double c = op.c[0], sn = op.c[1];
+        const bool form_a = std::fabs(c) >= std::fabs(sn);
+        const double ratio[1] = {form_a ? sn / c : c / sn};
+        const double f = form_a ? c : sn;
+        gre *= f;
+        gim *= f;
+        const size_t k = cfix(ratio, 1);
+        std::string arg = cf_str(k);
+        if (op.kind == G_RY) {
+          // X Ry(theta) X = Ry(-theta): the ratio changes sign, and in form b
+          // so does the factor taken out.
+          arg = sel(tlp, "-" + cf_str(k), cf_str(k));
+          if (!form_a && framed[(size_t)tlp]) o << "gneg ^= " << fname(tlp) << "; ";
+        }
+        o << (op.kind == G_RX ? "qj_rx" : "qj_ry") << (form_a ? "a" : "b") << tmpl << op.rbit << ">(vr, vi, " << arg
+          << ");";
+      } else if (factor_diag(op)) {
+        // diag(A, D) = A diag(1, D/A), |A| = 1 for the reference's diagonal kinds
+        // but divided exactly as complex numbers in general.
+        const double ar = op.c[0], ai = op.c[1], dr = op.c[6], di = op.c[7];
+        const double den = ar * ar + ai * ai;
+        const double q[2] = {(dr * ar + di * ai) / den, (di * ar - dr * ai) / den};
+        const double nr = gre * ar - gim * ai, ni = gre * ai + gim * ar;
+        gre = nr;
+        gim = ni;
+        const size_t k = cfix(q, 2);
+        const int b = __builtin_ctz(op.dreg);
+        if (framed[(size_t)tlp])
+          o << "qj_diag" << tmpl << b << ", 0, true>(vr, vi, " << sel(tlp, cf_str(k), "(F)1") << ", "
+            << sel(tlp, cf_str(k + 1), "(F)0") << ", " << sel(tlp, "(F)1", cf_str(k)) << ", "
+            << sel(tlp, "(F)0", cf_str(k + 1)) << ");";
+        else
+          o << "qj_diag" << tmpl << b << ", 0, false>(vr, vi, (F)1, (F)0, " << cf_str(k) << ", " << cf_str(k + 1)
+            << ");";
+      } else if (op.rbit >= 0) {
         const std::string bm = std::to_string(op.rbit) + ", " + m + ">";
         switch (op.kind) {
           case G_RX: {
             const size_t k = cfix(op.c, 2);
-            o << "qj_rx" << tmpl << bm << "(vr, vi, (F)cf[" << k << "], (F)cf[" << k + 1 << "]);";
+            o << "qj_rx" << tmpl << bm << "(vr, vi, " << cf_str(k) << ", " << cf_str(k + 1) << ");";
             break;
           }
           case G_RY: {
             const size_t k = cfix(op.c, 2);
-            o << "qj_ry" << tmpl << bm << "(vr, vi, (F)cf[" << k << "], (F)cf[" << k + 1 << "]);";
+            o << "qj_ry" << tmpl << bm << "(vr, vi, " << cf_str(k) << ", "
+              << sel(tlp, "-" + cf_str(k + 1), cf_str(k + 1)) << ");";
             break;
           }
           case G_H: {
             const size_t k = cfix(op.c, 1);
-            o << "qj_h" << tmpl << bm << "(vr, vi, (F)cf[" << k << "]);";
+            o << "qj_h" << tmpl << bm << "(vr, vi, " << cf_str(k) << ");";
             break;
           }
           case G_Y:
@@ -330,53 +550,78 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       } else {
         const bool full = op.kind == G_DIAG;
         const size_t k = cfix(op.c, 8);
-        const std::string A = "(F)cf[" + std::to_string(k) + "], (F)cf[" + std::to_string(k + 1) + "]";
-        const std::string D = "(F)cf[" + std::to_string(k + 6) + "], (F)cf[" + std::to_string(k + 7) + "]";
+        const std::string ar = full ? cf_str(k) : "(F)1", ai = full ? cf_str(k + 1) : "(F)0";
+        const std::string dr = cf_str(k + 6), di = cf_str(k + 7);
         if (op.dreg) {
-          o << "qj_diag" << tmpl << __builtin_ctz(op.dreg) << ", " << m << ", " << (full ? "true" : "false")
-            << ">(vr, vi, " << A << ", " << D << ");";
+          const int b = __builtin_ctz(op.dreg);
+          if (framed[(size_t)tlp])
+            o << "qj_diag" << tmpl << b << ", " << m << ", true>(vr, vi, " << sel(tlp, dr, ar) << ", "
+              << sel(tlp, di, ai) << ", " << sel(tlp, ar, dr) << ", " << sel(tlp, ai, di) << ");";
+          else
+            o << "qj_diag" << tmpl << b << ", " << m << ", " << (full ? "true" : "false") << ">(vr, vi, " << ar
+              << ", " << ai << ", " << dr << ", " << di << ");";
         } else {
-          std::string hi;
-          if (op.dlane) hi = "(t & " + std::to_string(op.dlane) + "u) != 0u";
-          if (op.dout) hi += std::string(hi.empty() ? "" : " || ") + "(base & " + hex(op.dout) + ") != 0ull";
-          if (!full) {
-            o << "if (" << hi << ") qj_phase" << tmpl << m << ">(vr, vi, " << D << ");";
-          } else {
-            o << "const bool hi = " << hi << "; qj_phase" << tmpl << m << ">(vr, vi, hi ? (F)cf[" << k + 6
-              << "] : (F)cf[" << k << "], hi ? (F)cf[" << k + 7 << "] : (F)cf[" << k + 1 << "]);";
-          }
+          const std::string hi = hi_expr(op, ph);
+          if (!full)
+            o << "if (" << hi << ") qj_phase" << tmpl << m << ">(vr, vi, " << dr << ", " << di << ");";
+          else
+            o << "const bool hi = " << hi << "; qj_phase" << tmpl << m << ">(vr, vi, hi ? " << dr << " : " << ar
+              << ", hi ? " << di << " : " << ai << ");";
         }
       }
       o << " }\n";
     }
     const int dcnt = d.phase_diag_count[ph];
+    const bool last = ph == d.nphases - 1;
+    std::string g0 = "1", g1 = "0";
+    if (any_factor && last) {
+      g0 = cf_str(gslot), g1 = cf_str(gslot + 1);
+      if (use_gneg) g0 = "(gneg ? -" + g0 + " : " + g0 + ")", g1 = "(gneg ? -" + g1 + " : " + g1 + ")";
+    }
+    if (!dcnt && any_factor && last)
+      o << "    qj_phase" << tmpl << "0>(vr, vi, " << g0 << ", " << g1 << ");\n";
     if (dcnt) {
-      // Merged lane/outside diagonals: one per-lane phase, one multiplication.
-      o << "    { F pr = 1, pi = 0;\n";
+      // Merged lane/outside diagonals: one per-lane phase, one multiplication
+      // (the last phase's starts from the factors taken out of the pass).
+      o << "    { F pr = " << g0 << ", pi = " << g1 << ";\n";
       for (int i = 0; i < dcnt; i++) {
         const TileOp &q = ops[d.phase_diag_begin[ph] + i];
         const size_t k = cfix(q.c, 8);
         const bool full = q.kind == G_DIAG;
         std::string on = "true";
-        if (q.lane_cmask || q.out_cmask) on = cond(q);
-        std::string hi = "false";
-        if (q.dlane || q.dout) {
-          hi.clear();
-          if (q.dlane) hi = "(t & " + std::to_string(q.dlane) + "u) != 0u";
-          if (q.dout) hi += std::string(hi.empty() ? "" : " || ") + "(base & " + hex(q.dout) + ") != 0ull";
-        }
-        o << "      { const bool on = " << on << "; const bool hi = " << hi << ";\n";
-        o << "        const F fr = on ? (hi ? (F)cf[" << k + 6 << "] : " << (full ? "(F)cf[" + std::to_string(k) + "]" : "(F)1")
+        if (q.lane_cmask || q.out_cmask) on = cond(q, ph);
+        o << "      { const bool on = " << on << "; const bool hi = " << hi_expr(q, ph) << ";\n";
+        o << "        const F fr = on ? (hi ? " << cf_str(k + 6) << " : " << (full ? cf_str(k) : "(F)1")
           << ") : (F)1;\n";
-        o << "        const F fi = on ? (hi ? (F)cf[" << k + 7 << "] : " << (full ? "(F)cf[" + std::to_string(k + 1) + "]" : "(F)0")
+        o << "        const F fi = on ? (hi ? " << cf_str(k + 7) << " : " << (full ? cf_str(k + 1) : "(F)0")
           << ") : (F)0;\n";
         o << "        const F nr = pr * fr - pi * fi, ni = pr * fi + pi * fr; pr = nr; pi = ni; }\n";
       }
       o << "      qj_phase" << tmpl << "0>(vr, vi, pr, pi); }\n";
     }
   }
-  if (!d.store_direct) transpose(cur, kIoStore);
-  o << "    { " << T << " *dst = s + (base | sl);\n";
+  if (any_factor) {
+    coef[gslot] = gre;
+    coef[gslot + 1] = gim;
+  }
+  if (!d.store_direct) move_to(cur, kIoStore);
+  // Pending exchanges at the store: a thread bit flips the store address, a
+  // register bit is performed on the registers.
+  const int st = d.store_direct ? cur : kIoStore;
+  std::string flips;
+  for (int lp = 0; lp < K; lp++) {
+    if (!framed[(size_t)lp]) continue;
+    int lane = -1;
+    for (int k = 0; k < L; k++)
+      if (d.lanepos[st][k] == lp) lane = k;
+    if (lane >= 0) {
+      flips += " ^ (" + fname(lp) + " ? " + hex(1ull << d.store_lane_gbit[lane]) + " : 0ull)";
+    } else {
+      for (int b = 0; b < R; b++)
+        if (lp_reg(st, b) == lp) settle(st, b);
+    }
+  }
+  o << "    { " << T << " *dst = s + (base | (sl" << flips << "));\n";
   for (int r = 0; r < N; r++) {
     if (nontemporal)
       o << "      { " << T << " *a = dst + " << hex(d.store_reg_off[r]) << "; __builtin_nontemporal_store(vr[" << r

@@ -15,7 +15,8 @@ from test_plan_host import random_all_kinds
 
 
 @pytest.mark.parametrize("n,precision,K,L,kind", [(14, 0, 12, 8, "layer"), (13, 0, 12, 8, "all"),
-                                                   (13, 1, 13, 8, "all"), (12, 0, 12, 9, "all")])
+                                                   (13, 1, 13, 8, "all"), (12, 0, 12, 9, "all"),
+                                                   (15, 0, 13, 8, "all"), (16, 0, 13, 8, "layer")])
 def test_generated_pass_kernels_compile(n, precision, K, L, kind):
     circuit = workloads.random_layer_circuit(n, 3, 1) if kind == "layer" else random_all_kinds(n, 90, n)
     ops = circuit.to_ops()
