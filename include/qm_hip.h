@@ -232,11 +232,12 @@ int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n
                   int32_t *passes, int32_t *transposes, int64_t *gates);
 /* Runs the fused schedule for an arbitrary tile shape on the host with the
  * kernels' exact arithmetic and indexing (complex128, qubits <= 26), so the
- * planner is verified without a device.  remap = 1 lets passes move qubits
- * into and out of the `contiguous` lowest index bits (the state is back in
- * natural order at the end). */
+ * planner is verified without a device.  flags: bits 0-1 the remapping mode
+ * (1 or 2: passes move qubits into and out of the `contiguous` lowest index
+ * bits; the state is back in natural order at the end), bit 2 the tile search
+ * (each pass's tile grown for the most runnable gates). */
 int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
-                       int32_t lane_bits, int32_t contiguous, int32_t remap, double *amplitudes);
+                       int32_t lane_bits, int32_t contiguous, int32_t flags, double *amplitudes);
 
 /* Decisions on CDF and branch boundaries: draws within the rigorous rounding
  * margin of a boundary were re-decided with the reference's sequential

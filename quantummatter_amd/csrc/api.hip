@@ -148,6 +148,11 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
       return e ? std::atoi(e) : -1;
     }();
     if (remap >= 0 && remap <= 2) shape.remap = remap;
+    static const int search = [] {
+      const char *e = std::getenv("QM_SEARCH");  // tuning knob: tile search 0/1
+      return e ? std::atoi(e) : -1;
+    }();
+    if (search == 0 || search == 1) shape.search = search;
   }
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
@@ -170,8 +175,11 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
     }
     schedule = QM_SCHED_FUSED;
   }
-  if (schedule == QM_SCHED_FUSED && !fused_shape_supported(s->precision, shape.L, shape.R))
+  if (schedule == QM_SCHED_FUSED && !fused_shape_supported(s->precision, shape.L, shape.R)) {
+    const int search = shape.search;
     shape = default_fused_shape(s->precision);
+    shape.search = search;
+  }
   if (schedule == QM_SCHED_FUSED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     std::vector<TileOp> flat = plan.flat_ops();
@@ -936,14 +944,15 @@ int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n
 }
 
 int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
-                       int32_t lane_bits, int32_t contiguous, int32_t remap, double *amps) {
+                       int32_t lane_bits, int32_t contiguous, int32_t flags, double *amps) {
   return guarded([&] {
     if (qubits < 1 || qubits > 26) fail(QM_ERR_RANGE, "host reference is limited to 26 qubits");
     if (!amps) fail(QM_ERR_ARG, "amplitudes are null");
     std::vector<DevGate> g = prepare_all(ops, n_ops, qubits);
     FusedShape shape{tile_bits, lane_bits, tile_bits - lane_bits, contiguous};
-    shape.remap = remap;
-    if (remap < 0 || remap > 2) fail(QM_ERR_RANGE, "remap must be 0, 1 or 2");
+    shape.remap = flags & 3;
+    shape.search = (flags >> 2) & 1;
+    if (flags < 0 || flags > 7 || shape.remap == 3) fail(QM_ERR_RANGE, "unsupported planning flags");
     if (shape.R < 1 || shape.R > 5 || shape.L < 1 || shape.L > 10 || shape.K > kMaxK ||
         contiguous < 0 || contiguous >= shape.K)
       fail(QM_ERR_RANGE, "unsupported tile shape");

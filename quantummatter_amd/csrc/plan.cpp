@@ -349,6 +349,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         if (logi[p] != p) q |= (uint64_t)1 << p;
       for (int x = 0; popc(q) < K && x < n; x++) q |= (uint64_t)1 << x;
     };
+    const uint64_t forced = Q;
     scan_tile(Q, true);
     if (popc(Q) < K && !deferred.empty()) {
       for (int idx : deferred) {
@@ -359,6 +360,48 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
       scan_tile(Q, false);
     }
     fill(Q);
+    if (sh.search) {
+      // Tile search: grow the tile one qubit at a time, each time by the qubit
+      // whose addition lets the most pending gates run in this pass (ties: the
+      // qubit a pending gate needs first); keep it if it beats the in-order
+      // choice above.
+      auto runnable = [&](uint64_t q) {
+        uint64_t blocked = 0;
+        int count = 0;
+        for (size_t k = 0; k < pending.size() && blocked != all; k++) {
+          const PGate &x = G[pending[k]];
+          if ((x.touch & blocked) || (x.pair && !(q & ((uint64_t)1 << x.g.target)))) {
+            blocked |= x.touch;
+            continue;
+          }
+          count++;
+        }
+        return count;
+      };
+      int first_need[64];
+      for (int x = 0; x < 64; x++) first_need[x] = INT32_MAX;
+      for (size_t k = 0; k < pending.size(); k++) {
+        const PGate &x = G[pending[k]];
+        if (x.pair && first_need[x.g.target] == INT32_MAX) first_need[x.g.target] = (int)k;
+      }
+      uint64_t S = forced;
+      while (popc(S) < K) {
+        int best = -1, best_count = -1;
+        for (int x = 0; x < n; x++) {
+          if (S & ((uint64_t)1 << x)) continue;
+          const int cnt = runnable(S | ((uint64_t)1 << x));
+          if (cnt > best_count || (cnt == best_count && first_need[x] < first_need[best])) {
+            best = x;
+            best_count = cnt;
+          }
+        }
+        S |= (uint64_t)1 << best;
+      }
+      if (runnable(S) > (int)included.size()) {
+        Q = S;
+        scan_tile(Q, false);
+      }
+    }
 
     int phys[64];
     for (int x = 0; x < 64; x++) phys[logi[x]] = x;

@@ -185,6 +185,9 @@ struct FusedShape {
   // last phase cannot store directly); 2: as 1, but only moves that keep the
   // store direct (and the final restore).
   int remap = 0;
+  // 1: each pass's tile is also grown qubit by qubit for the most runnable
+  // gates, and the better of the two choices is taken (slower planning).
+  int search = 0;
 };
 
 struct FusedPass {

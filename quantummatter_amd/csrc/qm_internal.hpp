@@ -55,10 +55,12 @@ int guarded(Fn &&fn) {
 }
 
 // Default form of a compiled-schedule shape: 2^K-amplitude tiles over 2^L
-// threads, the lowest 4 index bits in every tile, remapped between passes.
+// threads, the lowest 4 index bits in every tile, remapped between passes,
+// tiles chosen by the tile search.
 inline FusedShape compiled_shape(int K, int L) {
   FusedShape s{K, L, K - L, 4};
   s.remap = 1;
+  s.search = 1;
   return s;
 }
 

@@ -71,7 +71,7 @@ def test_fused_plan_matches_oracle_all_kinds(shape, n):
 # index bits (1: with a transpose before the store when needed, 2: only where
 # the store stays direct).
 REMAP_SHAPES = [(12, 8, 6, 1), (13, 8, 4, 1), (13, 8, 5, 2), (10, 6, 4, 1), (8, 5, 3, 2), (9, 4, 4, 1),
-                (6, 3, 3, 1)]
+                (6, 3, 3, 1), (13, 8, 4, 1, True), (12, 8, 3, 0, True), (10, 6, 4, 2, True), (8, 5, 3, 1, True)]
 
 
 @pytest.mark.parametrize("shape", REMAP_SHAPES)
@@ -97,7 +97,8 @@ def test_remapped_plan_layer_circuit(n):
     c = workloads.random_layer_circuit(n, 12, seed=78)
     ops = c.to_ops()
     exp = oracle.execute(n, ops)
-    for shape in [(12, 8, 6, 1), (13, 8, 6, 1), (13, 8, 4, 2), (12, 7, 5, 1)]:
+    for shape in [(12, 8, 6, 1), (13, 8, 6, 1), (13, 8, 4, 2), (12, 7, 5, 1), (13, 8, 4, 1, True),
+                  (12, 8, 4, 0, True)]:
         if n < shape[0]:
             continue
         got = plan_apply_host(n, ops, oracle.zero_state(n), *shape)
