@@ -112,6 +112,22 @@ std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
 }
 
 
+// Tuning knobs read from the environment (unset: no change): QM_CONTIGUOUS
+// (lowest index bits kept in every tile), QM_REMAP (remapping mode 0..2),
+// QM_SEARCH (tile search 0/1), QM_MAX_PHASES (phases per pass at most).
+void apply_tuning_knobs(FusedShape &shape) {
+  auto knob = [](const char *name) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : -1;
+  };
+  static const int contiguous = knob("QM_CONTIGUOUS"), remap = knob("QM_REMAP"), search = knob("QM_SEARCH"),
+                   max_phases = knob("QM_MAX_PHASES");
+  if (contiguous >= 0 && contiguous < shape.K) shape.contiguous = contiguous;
+  if (remap >= 0 && remap <= 2) shape.remap = remap;
+  if (search == 0 || search == 1) shape.search = search;
+  if (max_phases >= 1) shape.max_phases = max_phases;
+}
+
 // Runs prepared gates on a resident state with the chosen schedule.
 ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in,
                     const volatile int32_t *cancel, const FusedShape *force) {
@@ -129,31 +145,18 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
     const FusedShape wide = shape;
     shape = FusedShape{12, 8, 4, wide.contiguous};  // narrower states: 4096-amplitude tiles
     shape.remap = wide.remap;
+    shape.search = wide.search;
+    shape.max_phases = wide.max_phases;
   }
   if (!force && schedule == QM_SCHED_FUSED && !(tuned_slot >= 0 && c->tuned[0][s->precision][s->qubits].K)) {
     // The interpreting kernel keeps the lowest qubits in place (its passes are
-    // bound by the kernel's own work, not by the access pattern).
+    // bound by the kernel's own work, not by the access pattern); its tiles are
+    // searched like the compiled schedule's.
     shape.contiguous = s->precision == QM_C128 ? 3 : 4;
     shape.remap = 0;
+    shape.search = 1;
   }
-  if (!force) {
-    // Tuning knob: lowest qubits kept in every tile (longer contiguous runs).
-    static const int contiguous = [] {
-      const char *e = std::getenv("QM_CONTIGUOUS");
-      return e ? std::atoi(e) : -1;
-    }();
-    if (contiguous >= 0 && contiguous < shape.K) shape.contiguous = contiguous;
-    static const int remap = [] {
-      const char *e = std::getenv("QM_REMAP");  // tuning knob: remapping mode 0..2
-      return e ? std::atoi(e) : -1;
-    }();
-    if (remap >= 0 && remap <= 2) shape.remap = remap;
-    static const int search = [] {
-      const char *e = std::getenv("QM_SEARCH");  // tuning knob: tile search 0/1
-      return e ? std::atoi(e) : -1;
-    }();
-    if (search == 0 || search == 1) shape.search = search;
-  }
+  if (!force) apply_tuning_knobs(shape);
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     if (jit_supported(plan)) {
@@ -802,7 +805,9 @@ int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, in
       fail(QM_ERR_UNSUPPORTED, "the compiled schedule does not generate this tile shape");
     if (qubits < tile_bits || qubits > 34) fail(QM_ERR_RANGE, "qubits");
     std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
-    FusedPlan plan = plan_fused(gates, qubits, compiled_shape(tile_bits, lane_bits));
+    FusedShape shape = compiled_shape(tile_bits, lane_bits);
+    apply_tuning_knobs(shape);
+    FusedPlan plan = plan_fused(gates, qubits, shape);
     *passes = jit_compile_check(plan, precision);
   });
 }

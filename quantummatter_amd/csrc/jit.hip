@@ -217,19 +217,25 @@ std::string hex(uint64_t v) {
 }
 
 // Source of one pass kernel, and the coefficient layout it reads.
-// Code-generation variant (tuning knob QM_JIT_VARIANT, default 17): bit 0
+// Code-generation variant (tuning knob QM_JIT_VARIANT, default 1): bit 0
 // exchanges through an LDS buffer of reals (half the LDS, real then imaginary
 // parts, so more workgroups fit a CU), bit 2 forms the lane part of each LDS
 // address at its transpose instead of once per kernel, bit 3 loads and stores
 // the state with non-temporal hints (a pass never re-reads what it wrote),
 // bit 4 takes the common factor out of uncontrolled rotations and register
 // diagonals (ratio forms, one multiply-add per output) and applies the
-// product of the factors once per pass, bit 5 keeps exchanges conditioned on
-// thread or outside bits as per-tile flags (X frames, see pass_source).
+// product of the factors once per pass (off by default: on the device some
+// of its generated passes computed wrong states, 2 to 3 of 60 layer circuits
+// of 21..30 qubits, while the same sources are exact in the host emulation;
+// the failing passes change with the optimisation level and the cause is not
+// identified), bit 5 keeps exchanges conditioned on
+// thread or outside bits as per-tile flags (X frames, see pass_source), bit 6
+// issues the next tile's loads before the current tile's work (compiled for
+// one workgroup per CU, so the registers of both tiles fit).
 int jit_variant() {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
-    return e ? std::atoi(e) : 17;
+    return e ? std::atoi(e) : 1;
   }();
   return v;
 }
@@ -246,6 +252,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const int variant = variant_of(sh);
   const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
   const bool nontemporal = (variant & 8) != 0, factored = (variant & 16) != 0, frames = (variant & 32) != 0;
+  const bool prefetch = (variant & 64) != 0;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -289,23 +296,44 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   };
   if (!inline_j)
     for (int slot : used) o << "  const unsigned j" << slot << " = " << lane_addr(slot) << ";\n";
-  o << "  for (unsigned long long tau = blockIdx.x; tau < tiles; tau += gridDim.x) {\n";
-  o << "    unsigned long long base = 0ull";
-  for (int k = 0; k < d.nruns; k++)
-    o << " | (((tau >> " << (int)d.run_src[k] << ") & " << hex((1ull << d.run_len[k]) - 1) << ") << "
-      << (int)d.run_dst[k] << ")";
-  o << ";\n";
-  o << "    F vr[" << N << "], vi[" << N << "];\n";
-  o << "    { const " << T << " *src = s + (base | ll);\n";
-  for (int r = 0; r < N; r++) {
-    if (nontemporal)
-      o << "      { const " << T << " *a = src + " << hex(d.load_reg_off[r]) << "; vr[" << r
-        << "] = __builtin_nontemporal_load(&a->re); vi[" << r << "] = __builtin_nontemporal_load(&a->im); }\n";
-    else
-      o << "      { const " << T << " a = src[" << hex(d.load_reg_off[r]) << "]; vr[" << r << "] = a.re; vi["
-        << r << "] = a.im; }\n";
+  auto base_expr = [&](const std::string &tv) {
+    std::string e = "0ull";
+    for (int k = 0; k < d.nruns; k++)
+      e += " | (((" + tv + " >> " + std::to_string((int)d.run_src[k]) + ") & " + hex((1ull << d.run_len[k]) - 1) +
+           ") << " + std::to_string((int)d.run_dst[k]) + ")";
+    return e;
+  };
+  auto loads = [&](const char *re, const char *im, const std::string &b) {
+    o << "    { const " << T << " *src = s + ((" << b << ") | ll);\n";
+    for (int r = 0; r < N; r++) {
+      if (nontemporal)
+        o << "      { const " << T << " *a = src + " << hex(d.load_reg_off[r]) << "; " << re << "[" << r
+          << "] = __builtin_nontemporal_load(&a->re); " << im << "[" << r << "] = __builtin_nontemporal_load(&a->im); }\n";
+      else
+        o << "      { const " << T << " a = src[" << hex(d.load_reg_off[r]) << "]; " << re << "[" << r << "] = a.re; "
+          << im << "[" << r << "] = a.im; }\n";
+    }
+    o << "    }\n";
+  };
+  if (prefetch) {
+    // Software pipelining: the next tile's loads are issued before this tile's
+    // work, so they are in flight during its exchanges and arithmetic.
+    o << "  F nr[" << N << "], ni[" << N << "];\n";
+    o << "  if (blockIdx.x < tiles) {\n";
+    loads("nr", "ni", base_expr("(unsigned long long)blockIdx.x"));
+    o << "  }\n";
   }
-  o << "    }\n";
+  o << "  for (unsigned long long tau = blockIdx.x; tau < tiles; tau += gridDim.x) {\n";
+  o << "    const unsigned long long base = " << base_expr("tau") << ";\n";
+  o << "    F vr[" << N << "], vi[" << N << "];\n";
+  if (prefetch) {
+    for (int r = 0; r < N; r++) o << "    vr[" << r << "] = nr[" << r << "]; vi[" << r << "] = ni[" << r << "];\n";
+    o << "    if (tau + gridDim.x < tiles) {\n";
+    loads("nr", "ni", base_expr("(tau + gridDim.x)"));
+    o << "    }\n";
+  } else {
+    loads("vr", "vi", "base");
+  }
   auto transpose = [&](int from, int to) {
     o << "    {\n";
     if (inline_j) {
@@ -791,7 +819,14 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
     std::vector<size_t> todo(missing.size());
     for (size_t k = 0; k < todo.size(); k++) todo[k] = k;
     std::vector<Compiled> done(missing.size());
-    for (int occ = 3; occ >= 2 && !todo.empty(); occ--) {
+    const bool prefetch = (variant_of(plan.shape) & 64) != 0;
+    static const int fixed_occ = [] {
+      const char *e = std::getenv("QM_JIT_OCC");  // tuning knob: compile for this occupancy only
+      return e ? std::atoi(e) : 0;
+    }();
+    const int occ_hi = fixed_occ > 0 ? fixed_occ : (prefetch ? 1 : 3);
+    const int occ_lo = fixed_occ > 0 ? fixed_occ : (prefetch ? 1 : 2);
+    for (int occ = occ_hi; occ >= occ_lo && !todo.empty(); occ--) {
       std::vector<std::string> srcs;
       for (size_t k : todo) {
         const std::string &key = keys[missing[k]];
@@ -805,7 +840,7 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
         QM_HIP(hipModuleGetFunction(&cp.fn, cp.mod, "qm_pass"));
         int scratch = 0;
         QM_HIP(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, cp.fn));
-        if (scratch > 0 && occ > 2) {
+        if (scratch > 0 && occ > occ_lo) {
           (void)hipModuleUnload(cp.mod);
           again.push_back(todo[i]);
           continue;

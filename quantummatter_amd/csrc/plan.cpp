@@ -452,7 +452,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     std::vector<uint32_t> rsets;
     std::vector<std::vector<int>> phase_gates;
     while (!remaining.empty()) {
-      if ((int)rsets.size() == kMaxPhases) {
+      if ((int)rsets.size() == (sh.max_phases > 0 ? std::min(sh.max_phases, kMaxPhases) : kMaxPhases)) {
         overflow = remaining;
         break;
       }

@@ -188,6 +188,9 @@ struct FusedShape {
   // 1: each pass's tile is also grown qubit by qubit for the most runnable
   // gates, and the better of the two choices is taken (slower planning).
   int search = 0;
+  // Phases per pass at most (0: kMaxPhases).  Bounds the work of one pass, and
+  // with it the registers its generated kernel needs.
+  int max_phases = 0;
 };
 
 struct FusedPass {

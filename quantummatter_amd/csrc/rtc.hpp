@@ -5,6 +5,7 @@
 
 #include <hip/hiprtc.h>
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -18,8 +19,9 @@ inline std::vector<char> rtc_compile(const std::string &src, std::string &err) {
     err = "hiprtcCreateProgram failed";
     return {};
   }
-  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-Xclang", "-target-feature",
-                        "-Xclang", "-packed-fp32-ops"};
+  const char *level = std::getenv("QM_JIT_OPT");  // tuning knob: optimisation level (default -O3)
+  const char *opts[] = {"--offload-arch=gfx950", level && *level ? level : "-O3", "-ffp-contract=off", "-Xclang",
+                        "-target-feature", "-Xclang", "-packed-fp32-ops"};
   const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
   std::vector<char> code;
   if (rc != HIPRTC_SUCCESS) {

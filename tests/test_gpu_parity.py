@@ -307,3 +307,18 @@ def test_dense_blocks_layer_circuit_16q():
         s.EvolveDense(c, 5)
         got = s.Snapshot()
     assert np.max(np.abs(got - want)) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,layers,seed", [(25, 40, 2), (28, 40, 1), (29, 12, 1)])
+def test_compiled_schedule_wide_layer_circuits(n, layers, seed):
+    """Searched, remapped compiled passes at widths where every tile holds a
+    minority of the qubits, against the per-gate schedule (exact arithmetic of
+    the reference): the states agree to rounding."""
+    from quantummatter_amd.backend import SCHED_COMPILED, SCHED_PER_GATE
+    ops = workloads.random_layer_circuit(n, layers, seed=seed).to_ops()
+    with session(n) as a, session(n) as b:
+        a.EvolveOps(ops, SCHED_PER_GATE)
+        b.EvolveOps(ops, SCHED_COMPILED)
+        assert abs(1 - abs(b.Inner(a))) <= 1e-12
+        assert abs(b.Mass() - 1) <= 1e-12

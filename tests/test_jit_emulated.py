@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HARNESS_HEAD = r'''
 #include <barrier>
 #include <cmath>
+#include <memory>
 #include <cstdio>
 #include <cstdlib>
 #include <thread>
@@ -37,22 +38,26 @@ HARNESS_HEAD = r'''
 #define __shared__
 #define QM_OCC 2
 struct Dim { unsigned x; };
-thread_local Dim threadIdx;
-Dim blockIdx{0}, gridDim{1};
-std::barrier<> *g_bar;
+thread_local Dim threadIdx, blockIdx;
+Dim gridDim{1};
+thread_local std::barrier<> *g_bar;
 inline void __syncthreads() { g_bar->arrive_and_wait(); }
-double lds[1 << 15];
+thread_local double *g_lds;
+#define lds g_lds
 '''
 
 HARNESS_MAIN = r'''
 int main(int argc, char **argv) {
   const int n = std::atoi(argv[1]), K = std::atoi(argv[2]), threads = std::atoi(argv[3]);
+  const int blocks = argc > 7 ? std::atoi(argv[7]) : 1, stop = argc > 8 ? std::atoi(argv[8]) : NPASS;
+  gridDim.x = (unsigned)blocks;
+  std::vector<std::vector<double>> shared(blocks, std::vector<double>(1 << 15));
   const size_t amps = (size_t)1 << n;
   std::vector<qj_c128> s(amps);
   FILE *f = std::fopen(argv[4], "rb");
   if (std::fread(s.data(), sizeof(qj_c128), amps, f) != amps) return 2;
   std::fclose(f);
-  for (int i = 0; i < NPASS; i++) {
+  for (int i = 0; i < stop; i++) {
     std::vector<double> cf;
     char path[4096];
     std::snprintf(path, sizeof path, "%s/pass%d.coef", argv[5], i);
@@ -61,14 +66,18 @@ int main(int argc, char **argv) {
     while (c && std::fscanf(c, "%lf", &v) == 1) cf.push_back(v);
     if (c) std::fclose(c);
     cf.push_back(0.0);
-    std::barrier<> bar(threads);
-    g_bar = &bar;
+    std::vector<std::unique_ptr<std::barrier<>>> bars;
+    for (int b = 0; b < blocks; b++) bars.emplace_back(new std::barrier<>(threads));
     std::vector<std::thread> th;
-    for (int t = 0; t < threads; t++)
-      th.emplace_back([&, t] {
-        threadIdx.x = (unsigned)t;
-        KERNELS[i](s.data(), cf.data(), (unsigned long long)(amps >> K));
-      });
+    for (int b = 0; b < blocks; b++)
+      for (int t = 0; t < threads; t++)
+        th.emplace_back([&, b, t] {
+          threadIdx.x = (unsigned)t;
+          blockIdx.x = (unsigned)b;
+          g_bar = bars[b].get();
+          g_lds = shared[b].data();
+          KERNELS[i](s.data(), cf.data(), (unsigned long long)(amps >> K));
+        });
     for (auto &x : th) x.join();
   }
   f = std::fopen(argv[6], "wb");
@@ -84,7 +93,7 @@ def _prelude():
     return src.split('R"QMJIT(')[1].split(')QMJIT"')[0]
 
 
-def run_generated(tmp_path, n, ops, psi, K=13, L=8):
+def run_generated(tmp_path, n, ops, psi, K=13, L=8, blocks=1, stop=None):
     dump = tmp_path / "dump"
     dump.mkdir()
     os.environ["QM_JIT_DUMP"] = str(dump)
@@ -113,7 +122,10 @@ def run_generated(tmp_path, n, ops, psi, K=13, L=8):
     assert r.returncode == 0, r.stderr[-3000:]
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     np.ascontiguousarray(psi, dtype=np.complex128).tofile(inp)
-    r = subprocess.run([str(exe), str(n), str(K), str(1 << L), str(inp), str(dump), str(out)],
+    args = [str(exe), str(n), str(K), str(1 << L), str(inp), str(dump), str(out), str(blocks)]
+    if stop is not None:
+        args.append(str(stop))
+    r = subprocess.run(args,
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
     return np.fromfile(out, dtype=np.complex128), npass
@@ -166,8 +178,8 @@ def _variant_case(variant, kind, n, seed, tmp_path):
 
 
 @pytest.mark.parametrize("variant,kind,n,seed", [(49, "all", 14, 3), (49, "all", 15, 1003), (49, "layer", 16, 6),
-                                                 (1, "all", 14, 4)])
+                                                 (17, "all", 14, 4), (17, "layer", 16, 7)])
 def test_generated_kernel_variants_match_oracle(tmp_path, variant, kind, n, seed):
-    """Other code variants: exchange flags (bit 5, with factored forms) and the
-    plain forms (no factoring)."""
+    """Optional code variants: factored forms (bit 4) and exchange flags with
+    factored forms (bits 4 and 5)."""
     assert _variant_case(variant, kind, n, seed, tmp_path) <= 1e-12
