@@ -7,9 +7,12 @@ execution of the full 40-layer circuit on the resident state.  `value` is
 gate-layers per second over the timed steps (wall clock around K steps, each
 synchronised), with the state already in HBM.
 
-The passes run as compiled straight-line kernels (one per planned pass,
-compiled during the warmup and cached by circuit structure; compile time is
-reported in config), or with --schedule fused through the interpreting kernel.
+The passes (tiles chosen by the planner's tile search, the lowest index bits
+remapped between passes) run as compiled straight-line kernels (one per
+planned pass, compiled during the warmup and cached by circuit structure;
+compile time is reported in config), or with --schedule fused through the
+interpreting kernel.  The same circuit on a complex64 state is reported under
+"c64_compiled".
 Also reported: the dominant kernel's roofline (the tile pass: 2 x 16 B x
 2^30 algorithmic bytes per launch, average duration from HIP events recorded
 by the library on its stream around the timed sequence), and the CPU baseline
@@ -305,6 +308,18 @@ def main():
                     "gate_layers_per_s": 2 / (p2.DeviceEvolutionMs * 1e-3),
                     "launches_per_layer": p2.KernelLaunches / 2,
                     "algorithmic_GBps": p2.AlgorithmicBytes / (p2.DeviceEvolutionMs * 1e-3) / 1e9,
+                }
+        if not args.no_extras and sched == SCHED_COMPILED and prec == C128:
+            # The same circuit on a complex64 state (8 GiB), compiled passes
+            # (FP32 arithmetic; its infidelity bound is tested, not timed, here).
+            with be.CreateSession(n, C64) as s64:
+                s64.EvolveOps(ops, sched)  # compiles the c64 kernels
+                s64.Reset()
+                p64 = s64.EvolveOps(ops, sched, profile=True)
+                extras["c64_compiled"] = {
+                    "gate_layers_per_s": L / (p64.DeviceEvolutionMs * 1e-3),
+                    "passes": p64.Passes, "ms_per_pass": p64.DeviceEvolutionMs / p64.Passes,
+                    "algorithmic_GBps": p64.AlgorithmicBytes / (p64.DeviceEvolutionMs * 1e-3) / 1e9,
                 }
         if not args.no_extras and not args.no_batched:
             extras["batched"] = batched_extras(be)
