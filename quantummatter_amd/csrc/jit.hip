@@ -217,25 +217,28 @@ std::string hex(uint64_t v) {
 }
 
 // Source of one pass kernel, and the coefficient layout it reads.
-// Code-generation variant (tuning knob QM_JIT_VARIANT, default 1): bit 0
+// Code-generation variant (tuning knob QM_JIT_VARIANT, default 145 = bits 0,
+// 4 and 7): bit 0
 // exchanges through an LDS buffer of reals (half the LDS, real then imaginary
 // parts, so more workgroups fit a CU), bit 2 forms the lane part of each LDS
 // address at its transpose instead of once per kernel, bit 3 loads and stores
 // the state with non-temporal hints (a pass never re-reads what it wrote),
 // bit 4 takes the common factor out of uncontrolled rotations and register
 // diagonals (ratio forms, one multiply-add per output) and applies the
-// product of the factors once per pass (off by default: on the device some
-// of its generated passes computed wrong states, 2 to 3 of 60 layer circuits
-// of 21..30 qubits, while the same sources are exact in the host emulation;
-// the failing passes change with the optimisation level and the cause is not
-// identified), bit 5 keeps exchanges conditioned on
+// product of the factors once per pass (only with bit 7: without it, builds
+// of some of these passes computed wrong states on the device, 2 to 9 of 60
+// layer circuits of 21..30 qubits depending on optimisation level and
+// occupancy, while the same sources are exact in the host emulation; those
+// builds hold ~200 coefficients in scalar registers spilled to vector lanes,
+// and with bit 7 (no scalar spills) 140 such circuits are exact), bit 5 keeps exchanges conditioned on
 // thread or outside bits as per-tile flags (X frames, see pass_source), bit 6
 // issues the next tile's loads before the current tile's work (compiled for
-// one workgroup per CU, so the registers of both tiles fit).
+// one workgroup per CU, so the registers of both tiles fit), bit 7 loads the
+// coefficients inside the tile loop instead of once per kernel.
 int jit_variant() {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
-    return e ? std::atoi(e) : 1;
+    return e ? std::atoi(e) : 145;
   }();
   return v;
 }
@@ -252,7 +255,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const int variant = variant_of(sh);
   const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
   const bool nontemporal = (variant & 8) != 0, factored = (variant & 16) != 0, frames = (variant & 32) != 0;
-  const bool prefetch = (variant & 64) != 0;
+  const bool prefetch = (variant & 64) != 0, tile_coef = (variant & 128) != 0;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -263,7 +266,8 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   // when the kernel is compiled.
   (void)occ3;
   o << "extern \"C\" __global__ __launch_bounds__(" << (1 << L) << ", QM_OCC) void qm_pass(" << T
-    << " *__restrict__ s, const double *__restrict__ cf, unsigned long long tiles) {\n";
+    << " *__restrict__ s, const double *__restrict__ " << (tile_coef ? "cf_arg" : "cf")
+    << ", unsigned long long tiles) {\n";
   o << "  typedef " << F << " F;\n";
   if (split)
     o << "  extern __shared__ F lds[];\n";
@@ -324,6 +328,13 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     o << "  }\n";
   }
   o << "  for (unsigned long long tau = blockIdx.x; tau < tiles; tau += gridDim.x) {\n";
+  if (tile_coef) {
+    // The coefficient pointer is re-derived every tile (an empty asm that
+    // "changes" it), so coefficient loads stay next to their uses instead of
+    // being hoisted out of the loop and held in (spilled) scalar registers.
+    o << "    const double *cf = cf_arg;\n";
+    o << "    asm volatile(\"\" : \"+s\"(cf));\n";
+  }
   o << "    const unsigned long long base = " << base_expr("tau") << ";\n";
   o << "    F vr[" << N << "], vi[" << N << "];\n";
   if (prefetch) {

@@ -109,6 +109,7 @@ def run_generated(tmp_path, n, ops, psi, K=13, L=8, blocks=1, stop=None):
         k = (dump / f"pass{i}.hip").read_text()
         k = k.replace('extern "C" ', "").replace("void qm_pass(", f"void qm_pass_{i}(")
         k = re.sub(r"extern\s+__shared__\s+F\s+lds\[\];", "", k)
+        k = k.replace('asm volatile("" : "+s"(cf));', 'asm volatile("" : "+r"(cf));')  # host register class
         parts.append(k)
     parts.append(f"#define NPASS {npass}\ntypedef void (*Kernel)(qj_c128 *, const double *, unsigned long long);\n"
                  "Kernel KERNELS[] = {" +
@@ -178,7 +179,8 @@ def _variant_case(variant, kind, n, seed, tmp_path):
 
 
 @pytest.mark.parametrize("variant,kind,n,seed", [(49, "all", 14, 3), (49, "all", 15, 1003), (49, "layer", 16, 6),
-                                                 (17, "all", 14, 4), (17, "layer", 16, 7)])
+                                                 (17, "all", 14, 4), (17, "layer", 16, 7), (145, "all", 15, 9),
+                                                 (129, "layer", 16, 8)])
 def test_generated_kernel_variants_match_oracle(tmp_path, variant, kind, n, seed):
     """Optional code variants: factored forms (bit 4) and exchange flags with
     factored forms (bits 4 and 5)."""
