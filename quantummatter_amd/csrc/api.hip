@@ -143,7 +143,10 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
     shape = c->tuned[tuned_slot][s->precision][s->qubits];
   } else if (schedule == QM_SCHED_COMPILED && s->qubits < shape.K && s->qubits >= 12) {
     const FusedShape wide = shape;
-    shape = FusedShape{12, 8, 4, wide.contiguous};  // narrower states: 4096-amplitude tiles
+    // Narrower states: the widest tile of 256 threads that fits (8192 or 4096
+    // amplitudes).
+    const int k = std::min(s->qubits, 13);
+    shape = FusedShape{k, 8, k - 8, wide.contiguous};
     shape.remap = wide.remap;
     shape.search = wide.search;
     shape.max_phases = wide.max_phases;
@@ -800,7 +803,7 @@ int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, in
     if (!passes) fail(QM_ERR_ARG, "passes is null");
     if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
     const int R = tile_bits - lane_bits;
-    const int max_k = precision == QM_C128 ? 13 : 14;
+    const int max_k = 14;
     if (!(lane_bits >= 6 && lane_bits <= 10 && R >= 2 && R <= 5 && tile_bits <= max_k))
       fail(QM_ERR_UNSUPPORTED, "the compiled schedule does not generate this tile shape");
     if (qubits < tile_bits || qubits > 34) fail(QM_ERR_RANGE, "qubits");
@@ -831,9 +834,10 @@ int qm_ctx_set_fused_shape(qm_ctx *h, int32_t precision, int32_t tile_bits, int3
     if (precision != QM_C128 && precision != QM_C64) fail(QM_ERR_ARG, "unknown precision");
     const int R = tile_bits - lane_bits;
     // Any shape the compiled schedule can generate (its LDS exchange buffer holds
-    // 2^K reals within 64 KiB); the interpreting kernel uses this shape when it is
-    // one of its instantiations and its default shape otherwise.
-    const int max_k = precision == QM_C128 ? 13 : 14;
+    // 2^K reals: at most 128 KiB of the CU's 160 KiB); the interpreting kernel
+    // uses this shape when it is one of its instantiations and its default shape
+    // otherwise.
+    const int max_k = 14;
     if (!fused_shape_supported(precision, lane_bits, R) &&
         !(lane_bits >= 6 && lane_bits <= 10 && R >= 2 && R <= 5 && tile_bits <= max_k))
       fail(QM_ERR_UNSUPPORTED, "fused tile shape is not supported");
@@ -876,14 +880,16 @@ int qm_autotune(qm_ctx *h, int32_t qubits, int32_t precision, int32_t schedule, 
       for (const FusedShape &f : fused_shapes(precision))
         if (f.K <= qubits) cands.push_back(f);
     } else {
-      // Generated kernels: tiles of 2^11..2^13 (c128) or 2^12..2^14 (c64)
-      // amplitudes with 128..512 threads, in the default code variant.
+      // Generated kernels: tiles of 2^11..2^14 (c128) or 2^12..2^14 (c64)
+      // amplitudes with 128..1024 threads, in the default code variant.
       const int kl[][2] = {{11, 7}, {12, 7}, {12, 8}, {12, 9}, {13, 8}, {13, 9}, {14, 9}, {14, 10}};
       for (const auto &p : kl) {
         const int K = p[0], L = p[1], R = K - L;
-        if (K > qubits || R < 2 || R > 5 || (precision == QM_C128 && K > 13) || K < 12 - (precision == QM_C128))
+        if (K > qubits || R < 2 || R > 5 || K < 12 - (precision == QM_C128))
           continue;
         cands.push_back(compiled_shape(K, L));
+        if (K == 14 && L == 9)  // the default shape with 3 and 5 low bits kept in the tiles
+          for (int low : {3, 5}) cands.push_back(compiled_shape(K, L, low));
         if (K == 13 && L == 8) cands.push_back(FusedShape{K, L, R, contiguous});  // without remapping
       }
     }

@@ -217,45 +217,54 @@ std::string hex(uint64_t v) {
 }
 
 // Source of one pass kernel, and the coefficient layout it reads.
-// Code-generation variant (tuning knob QM_JIT_VARIANT, default 145 = bits 0,
-// 4 and 7): bit 0
-// exchanges through an LDS buffer of reals (half the LDS, real then imaginary
-// parts, so more workgroups fit a CU), bit 2 forms the lane part of each LDS
-// address at its transpose instead of once per kernel, bit 3 loads and stores
-// the state with non-temporal hints (a pass never re-reads what it wrote),
-// bit 4 takes the common factor out of uncontrolled rotations and register
-// diagonals (ratio forms, one multiply-add per output) and applies the
-// product of the factors once per pass (only with bit 7: without it, builds
-// of some of these passes computed wrong states on the device, 2 to 9 of 60
-// layer circuits of 21..30 qubits depending on optimisation level and
-// occupancy, while the same sources are exact in the host emulation; those
-// builds hold ~200 coefficients in scalar registers spilled to vector lanes,
-// and with bit 7 (no scalar spills) 140 such circuits are exact), bit 5 keeps exchanges conditioned on
-// thread or outside bits as per-tile flags (X frames, see pass_source), bit 6
-// issues the next tile's loads before the current tile's work (compiled for
-// one workgroup per CU, so the registers of both tiles fit), bit 7 loads the
-// coefficients inside the tile loop instead of once per kernel.
-int jit_variant() {
+// Code-generation variant (tuning knob QM_JIT_VARIANT; default 401 = bits 0,
+// 4, 7 and 8 in complex128, 913 = those and bit 9 in complex64): bit 0 exchanges through an LDS buffer of reals (half the LDS,
+// real then imaginary parts, so more workgroups fit a CU), bit 2 forms the
+// lane part of each LDS address at its transpose instead of once per kernel,
+// bit 3 loads and stores the state with non-temporal hints (a pass never
+// re-reads what it wrote), bit 4 takes the common factor out of uncontrolled
+// rotations and register diagonals (ratio forms, one multiply-add per output)
+// and applies the product of the factors once per pass (only with bit 7:
+// without it, builds of some of these passes computed wrong states on the
+// device, 2 to 9 of 60 layer circuits of 21..30 qubits depending on
+// optimisation level and occupancy, while the same sources are exact in the
+// host emulation; those builds hold ~200 coefficients in scalar registers
+// spilled to vector lanes, and with bit 7, no scalar spills, 140 such
+// circuits are exact), bit 5 keeps exchanges conditioned on thread or outside
+// bits as per-tile flags (X frames, see pass_source), bit 6 issues the next
+// tile's loads before the current tile's work (compiled for one workgroup per
+// CU, so the registers of both tiles fit), bit 7 loads the coefficients inside
+// the tile loop instead of once per kernel, bit 8 forms the swizzled LDS
+// addresses of each exchange there instead of holding those of every exchange
+// across the tile loop (which spilled up to 700 B per thread), bit 9 splits
+// each swizzled LDS address into the XOR of the few distinct parts that
+// overlap the lane part and an immediate offset (fewer address instructions
+// and paired LDS accesses; 6 % faster in complex64, 3 % slower in complex128
+// at 16384-amplitude tiles, where the extra live addresses spill).
+int jit_variant(int precision) {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
-    return e ? std::atoi(e) : 145;
+    return e ? std::atoi(e) : -1;
   }();
-  return v;
+  return v >= 0 ? v : precision == QM_C64 ? 913 : 401;
 }
 
-int variant_of(const FusedShape &sh) { return sh.variant >= 0 ? sh.variant : jit_variant(); }
+int variant_of(const FusedShape &sh, int precision) {
+  return sh.variant >= 0 ? sh.variant : jit_variant(precision);
+}
 
 size_t jit_lds_bytes(int precision, const FusedShape &sh) {
-  const bool split = (variant_of(sh) & 1) != 0;
+  const bool split = (variant_of(sh, precision) & 1) != 0;
   return (split ? (size_t)amp_bytes(precision) / 2 : (size_t)amp_bytes(precision)) << sh.K;
 }
 
 std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
                         std::vector<double> &coef) {
-  const int variant = variant_of(sh);
+  const int variant = variant_of(sh, precision);
   const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
   const bool nontemporal = (variant & 8) != 0, factored = (variant & 16) != 0, frames = (variant & 32) != 0;
   const bool prefetch = (variant & 64) != 0, tile_coef = (variant & 128) != 0;
+  const bool launder = (variant & 256) != 0, fold = (variant & 512) != 0;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -263,9 +272,12 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const char *T = c128 ? "qj_c128" : "qj_c64";
   std::ostringstream o;
   // QM_OCC (workgroups per CU the register allocation must allow) is defined
-  // when the kernel is compiled.
+  // when the kernel is compiled; the launch bound's second argument counts
+  // waves per SIMD, and a workgroup of 2^L threads puts 2^L / 256 waves on
+  // each of the CU's four SIMDs.
   (void)occ3;
-  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << L) << ", QM_OCC) void qm_pass(" << T
+  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << L) << ", (QM_OCC * " << (1 << L)
+    << " + 255) / 256) void qm_pass(" << T
     << " *__restrict__ s, const double *__restrict__ " << (tile_coef ? "cf_arg" : "cf")
     << ", unsigned long long tiles) {\n";
   o << "  typedef " << F << " F;\n";
@@ -351,23 +363,46 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       o << "    const unsigned j" << from << " = " << lane_addr(from) << ";\n";
       o << "    const unsigned j" << to << " = " << lane_addr(to) << ";\n";
     }
+    // With launder, the lane addresses are re-derived (an empty asm that
+    // "changes" them) before each exchange, so the 2 x N swizzled LDS
+    // addresses are computed there instead of being hoisted out of the tile
+    // loop and held, for every exchange of the pass at once, in (spilled)
+    // registers.
+    const std::string jw = launder ? "jw" : "j" + std::to_string(from);
+    const std::string jr = launder ? "jr" : "j" + std::to_string(to);
+    // Address of register r under a mapping: the lane part j XOR the
+    // register's constant c.  With fold, the bits of c outside every bit j can
+    // hold are added instead (same value, as j has none of them), so they
+    // become the LDS instruction's immediate offset, and only the few
+    // distinct XOR parts are computed.
+    auto at = [&](const std::string &j, int slot, unsigned c) {
+      if (!fold) return "lds[" + j + " ^ " + std::to_string(c) + "u]";
+      unsigned m = 0;
+      for (int k = 0; k < L; k++) m |= d.lane_swz[slot][k];
+      return "lds[(" + j + " ^ " + std::to_string(c & m) + "u) + " + std::to_string(c & ~m) + "u]";
+    };
     if (split) {
       for (const char *part : {"vr", "vi"}) {
         o << "    __syncthreads();\n";
+        if (launder) {
+          o << "    {\n    unsigned jw = j" << from << ", jr = j" << to << ";\n";
+          o << "    asm volatile(\"\" : \"+v\"(jw), \"+v\"(jr));\n";
+        }
         for (int r = 0; r < N; r++)
-          o << "    lds[j" << from << " ^ " << d.regswz[from][r] << "u] = " << part << "[" << r << "];\n";
+          o << "    " << at(jw, from, d.regswz[from][r]) << " = " << part << "[" << r << "];\n";
         o << "    __syncthreads();\n";
         for (int r = 0; r < N; r++)
-          o << "    " << part << "[" << r << "] = lds[j" << to << " ^ " << d.regswz[to][r] << "u];\n";
+          o << "    " << part << "[" << r << "] = " << at(jr, to, d.regswz[to][r]) << ";\n";
+        if (launder) o << "    }\n";
       }
     } else {
       o << "    __syncthreads();\n";
       for (int r = 0; r < N; r++)
-        o << "    { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; lds[j" << from << " ^ "
-          << d.regswz[from][r] << "u] = a; }\n";
+        o << "    { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; "
+          << at("j" + std::to_string(from), from, d.regswz[from][r]) << " = a; }\n";
       o << "    __syncthreads();\n";
       for (int r = 0; r < N; r++)
-        o << "    { const " << T << " a = lds[j" << to << " ^ " << d.regswz[to][r] << "u]; vr[" << r
+        o << "    { const " << T << " a = " << at("j" + std::to_string(to), to, d.regswz[to][r]) << "; vr[" << r
           << "] = a.re; vi[" << r << "] = a.im; }\n";
     }
     o << "    }\n";
@@ -825,18 +860,23 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
   }
   if (!missing.empty()) {
     const auto t0 = std::chrono::steady_clock::now();
-    // Highest occupancy (3, then 2 workgroups per CU) whose register
+    // Highest occupancy (3, then 2 workgroups per CU, at most what the
+    // exchange buffer leaves room for in the CU's LDS) whose register
     // allocation does not spill, per pass.
     std::vector<size_t> todo(missing.size());
     for (size_t k = 0; k < todo.size(); k++) todo[k] = k;
     std::vector<Compiled> done(missing.size());
-    const bool prefetch = (variant_of(plan.shape) & 64) != 0;
+    const bool prefetch = (variant_of(plan.shape, precision) & 64) != 0;
     static const int fixed_occ = [] {
       const char *e = std::getenv("QM_JIT_OCC");  // tuning knob: compile for this occupancy only
       return e ? std::atoi(e) : 0;
     }();
-    const int occ_hi = fixed_occ > 0 ? fixed_occ : (prefetch ? 1 : 3);
-    const int occ_lo = fixed_occ > 0 ? fixed_occ : (prefetch ? 1 : 2);
+    int lds_per_cu = 0;
+    QM_HIP(hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+    const int lds_occ =
+        std::max(1, (int)((size_t)lds_per_cu / std::max<size_t>(jit_lds_bytes(precision, plan.shape), 1)));
+    const int occ_hi = fixed_occ > 0 ? fixed_occ : std::min(prefetch ? 1 : 3, lds_occ);
+    const int occ_lo = fixed_occ > 0 ? fixed_occ : std::min(prefetch ? 1 : 2, lds_occ);
     for (int occ = occ_hi; occ >= occ_lo && !todo.empty(); occ--) {
       std::vector<std::string> srcs;
       for (size_t k : todo) {

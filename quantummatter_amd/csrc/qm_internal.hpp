@@ -55,10 +55,10 @@ int guarded(Fn &&fn) {
 }
 
 // Default form of a compiled-schedule shape: 2^K-amplitude tiles over 2^L
-// threads, the lowest 4 index bits in every tile, remapped between passes,
-// tiles chosen by the tile search.
-inline FusedShape compiled_shape(int K, int L) {
-  FusedShape s{K, L, K - L, 4};
+// threads, the lowest `contiguous` index bits (default 4) in every tile,
+// remapped between passes, tiles chosen by the tile search.
+inline FusedShape compiled_shape(int K, int L, int contiguous = 4) {
+  FusedShape s{K, L, K - L, contiguous};
   s.remap = 1;
   s.search = 1;
   return s;
@@ -89,10 +89,12 @@ struct Ctx {
   double last_dense_flops = 0;  // matrix-core flops of the last dense execution
   // Tile shape per precision (qm_ctx_set_fused_shape).  The compiled schedule
   // uses it as given; the interpreting kernel uses it when it is one of its
-  // instantiations and its own default otherwise.  Defaults: 8192-amplitude
-  // tiles, 256 threads x 32 registers, the low 4 index bits remapped between
-  // passes (the compiled schedule's best on the 30-qubit layer workload).
-  FusedShape shape[2] = {compiled_shape(13, 8), compiled_shape(13, 8)};
+  // instantiations and its own default otherwise.  Defaults: 16384-amplitude
+  // tiles, 512 threads x 32 registers (one workgroup per CU: the exchange
+  // buffer takes 128 KiB of LDS in complex128), the low 3 (complex128) or 4
+  // (complex64) index bits remapped between passes: the compiled schedule's
+  // best on the 30-qubit layer workload (14 and 18 passes for 40 layers).
+  FusedShape shape[2] = {compiled_shape(14, 9, 3), compiled_shape(14, 9, 4)};
   // Shapes chosen by qm_autotune, [fused 0 / compiled 1][precision][qubits];
   // K == 0: not tuned.  They take precedence over `shape` for their width.
   FusedShape tuned[2][2][35] = {};

@@ -109,7 +109,7 @@ def run_generated(tmp_path, n, ops, psi, K=13, L=8, blocks=1, stop=None):
         k = (dump / f"pass{i}.hip").read_text()
         k = k.replace('extern "C" ', "").replace("void qm_pass(", f"void qm_pass_{i}(")
         k = re.sub(r"extern\s+__shared__\s+F\s+lds\[\];", "", k)
-        k = k.replace('asm volatile("" : "+s"(cf));', 'asm volatile("" : "+r"(cf));')  # host register class
+        k = re.sub(r'"\+[sv]"\(', '"+r"(', k)  # empty asm operands: host register class
         parts.append(k)
     parts.append(f"#define NPASS {npass}\ntypedef void (*Kernel)(qj_c128 *, const double *, unsigned long long);\n"
                  "Kernel KERNELS[] = {" +
@@ -135,18 +135,20 @@ def run_generated(tmp_path, n, ops, psi, K=13, L=8, blocks=1, stop=None):
 pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 
 
-@pytest.mark.parametrize("kind,n,seed", [("all", 14, 3), ("all", 15, 8), ("layer", 16, 5)])
-def test_generated_kernels_match_oracle(tmp_path, kind, n, seed):
+@pytest.mark.parametrize("kind,n,seed,K,L", [("all", 14, 3, 13, 8), ("all", 15, 8, 13, 8), ("layer", 16, 5, 13, 8),
+                                             ("all", 15, 13, 14, 9), ("layer", 16, 14, 14, 9)])
+def test_generated_kernels_match_oracle(tmp_path, kind, n, seed, K, L):
     """Every gate kind (controls on register, thread and outside bits, swaps,
     Toffolis, diagonals) or the random layer workload, through the generated
-    kernels of the default code variant."""
+    kernels of the default code variant, for 8192-amplitude tiles of 256
+    threads and 16384-amplitude tiles of 512 threads."""
     c = random_all_kinds(n, 200, seed) if kind == "all" else workloads.random_layer_circuit(n, 10, seed)
     ops = c.to_ops()
     rng = np.random.default_rng(seed)
     psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
     psi /= np.linalg.norm(psi)
     want = oracle.execute(n, ops, psi)
-    got, npass = run_generated(tmp_path, n, ops, psi)
+    got, npass = run_generated(tmp_path, n, ops, psi, K=K, L=L)
     assert npass >= 1
     assert np.max(np.abs(got - want)) <= 1e-12
 
@@ -180,8 +182,11 @@ def _variant_case(variant, kind, n, seed, tmp_path):
 
 @pytest.mark.parametrize("variant,kind,n,seed", [(49, "all", 14, 3), (49, "all", 15, 1003), (49, "layer", 16, 6),
                                                  (17, "all", 14, 4), (17, "layer", 16, 7), (145, "all", 15, 9),
-                                                 (129, "layer", 16, 8)])
+                                                 (129, "layer", 16, 8), (401, "all", 15, 10),
+                                                 (401, "layer", 16, 11), (913, "all", 15, 12),
+                                                 (913, "layer", 16, 13)])
 def test_generated_kernel_variants_match_oracle(tmp_path, variant, kind, n, seed):
-    """Optional code variants: factored forms (bit 4) and exchange flags with
-    factored forms (bits 4 and 5)."""
+    """Optional code variants: factored forms (bit 4), exchange flags (bit 5),
+    per-tile coefficient loads (bit 7), per-exchange lane addresses (bit 8) and
+    folded LDS offsets (bit 9)."""
     assert _variant_case(variant, kind, n, seed, tmp_path) <= 1e-12
