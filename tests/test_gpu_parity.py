@@ -284,6 +284,36 @@ def test_compiled_schedule_c64_infidelity():
     assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-5
 
 
+def test_compiled_schedule_c64_all_kinds():
+    """Every gate kind through the complex64 generated kernels at the default
+    16384-amplitude tiles (folded LDS offsets), against the oracle."""
+    from quantummatter_amd.backend import SCHED_COMPILED
+    n = 15
+    c = random_all_kinds(n, 240, seed=515)
+    want = oracle.execute(n, c.to_ops())
+    with session(n, C64) as s:
+        s.Evolve(c, SCHED_COMPILED)
+        got = s.Snapshot()
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-5
+
+
+@pytest.mark.parametrize("precision", [C128, C64])
+def test_compiled_schedule_explicit_8192_tiles(precision):
+    """The previous default shape (8192-amplitude tiles of 256 threads, two
+    workgroups per CU), selected explicitly on a private context."""
+    from quantummatter_amd.backend import SCHED_COMPILED, HipStateVectorBackend
+    n = 17
+    c = workloads.random_layer_circuit(n, 12, seed=17)
+    want = oracle.execute(n, c.to_ops())
+    with HipStateVectorBackend() as be:
+        be.SetFusedShape(precision, 13, 8)
+        assert be.GetFusedShape(precision) == (13, 8)
+        with be.CreateSession(n, precision) as s:
+            s.Evolve(c, SCHED_COMPILED)
+            got = s.Snapshot()
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= (1e-5 if precision == C64 else 1e-12)
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("n", [10, 13])
 def test_dense_blocks_on_matrix_cores(n, k):
