@@ -133,7 +133,9 @@ int qm_ctx_resident_bytes(qm_ctx *ctx, int64_t *bytes);
 /* No reference counterpart (the reference has no fusion): the fused schedule's
  * tile shape per precision -- 2^tile_bits amplitudes per tile, 2^lane_bits
  * threads per workgroup.  Only compiled shapes are accepted (QM_ERR_UNSUPPORTED
- * otherwise).  Set by the caller's tuning or by qm_autotune. */
+ * otherwise): 6 <= lane_bits <= 10, 2 <= tile_bits - lane_bits <= 5,
+ * tile_bits <= 14.  Default 14/9 for both precisions.  Set by the caller's
+ * tuning or by qm_autotune. */
 int qm_ctx_set_fused_shape(qm_ctx *ctx, int32_t precision, int32_t tile_bits, int32_t lane_bits);
 int qm_ctx_get_fused_shape(qm_ctx *ctx, int32_t precision, int32_t *tile_bits, int32_t *lane_bits);
 
