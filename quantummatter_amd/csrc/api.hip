@@ -122,7 +122,7 @@ void apply_tuning_knobs(FusedShape &shape) {
   };
   static const int contiguous = knob("QM_CONTIGUOUS"), remap = knob("QM_REMAP"), search = knob("QM_SEARCH"),
                    max_phases = knob("QM_MAX_PHASES");
-  if (contiguous >= 0 && contiguous < shape.K) shape.contiguous = contiguous;
+  if (contiguous >= 0 && contiguous < shape.K) shape.contiguous = contiguous, shape.alt_contiguous = 0;
   if (remap >= 0 && remap <= 2) shape.remap = remap;
   if (search == 0 || search == 1) shape.search = search;
   if (max_phases >= 1) shape.max_phases = max_phases;
@@ -162,6 +162,19 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
   if (!force) apply_tuning_knobs(shape);
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
+    if (shape.alt_contiguous > 0 && shape.alt_contiguous < shape.K && shape.alt_contiguous != shape.contiguous) {
+      FusedShape alt = shape;
+      alt.contiguous = shape.alt_contiguous;
+      FusedPlan other = plan_fused(gates, s->qubits, alt);
+      auto transposes = [](const FusedPlan &p) {
+        int t = 0;
+        for (const FusedPass &x : p.passes) t += x.transposes;
+        return t;
+      };
+      if (other.passes.size() < plan.passes.size() ||
+          (other.passes.size() == plan.passes.size() && transposes(other) < transposes(plan)))
+        plan = std::move(other);
+    }
     if (jit_supported(plan)) {
       JitProgram jp = jit_prepare(plan, s->precision, c->device);
       const int64_t bytes = (int64_t)(jp.coef.size() * sizeof(double));

@@ -191,6 +191,11 @@ struct FusedShape {
   // Phases per pass at most (0: kMaxPhases).  Bounds the work of one pass, and
   // with it the registers its generated kernel needs.
   int max_phases = 0;
+  // Compiled schedule only (0: off): the circuit is also planned with this
+  // many low bits kept in the tiles, and the plan with fewer passes (then
+  // fewer transposes) runs.  The greedy tile search finds fewer passes with
+  // one or the other value circuit by circuit.
+  int alt_contiguous = 0;
 };
 
 struct FusedPass {

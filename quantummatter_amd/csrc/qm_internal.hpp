@@ -64,6 +64,11 @@ inline FusedShape compiled_shape(int K, int L, int contiguous = 4) {
   return s;
 }
 
+inline FusedShape alt_low_bits(FusedShape s, int alt) {
+  s.alt_contiguous = alt;
+  return s;
+}
+
 struct Ctx {
   int device = 0;
   bool profiling = false;
@@ -91,10 +96,11 @@ struct Ctx {
   // uses it as given; the interpreting kernel uses it when it is one of its
   // instantiations and its own default otherwise.  Defaults: 16384-amplitude
   // tiles, 512 threads x 32 registers (one workgroup per CU: the exchange
-  // buffer takes 128 KiB of LDS in complex128), the low 3 (complex128) or 5
-  // (complex64) index bits remapped between passes: the compiled schedule's
-  // best on the 30-qubit layer workload (14 and 19 passes for 40 layers).
-  FusedShape shape[2] = {compiled_shape(14, 9, 3), compiled_shape(14, 9, 5)};
+  // buffer takes 128 KiB of LDS in complex128), the low 3 (complex128, or 4
+  // when that plan has fewer passes) or 5 (complex64) index bits remapped
+  // between passes: the compiled schedule's best on the 30-qubit layer
+  // workload (14 and 19 passes for 40 layers).
+  FusedShape shape[2] = {alt_low_bits(compiled_shape(14, 9, 3), 4), compiled_shape(14, 9, 5)};
   // Shapes chosen by qm_autotune, [fused 0 / compiled 1][precision][qubits];
   // K == 0: not tuned.  They take precedence over `shape` for their width.
   FusedShape tuned[2][2][35] = {};

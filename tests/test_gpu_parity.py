@@ -339,6 +339,27 @@ def test_dense_blocks_layer_circuit_16q():
     assert np.max(np.abs(got - want)) <= 1e-12
 
 
+def test_compiled_schedule_plans_with_both_low_bit_counts():
+    """The complex128 compiled schedule plans with 3 and with 4 low bits kept
+    in the tiles and runs the plan with fewer passes: never more passes than
+    the 4-bit plan alone (qm_jit_compile_check plans with 4), and the same
+    state as the per-gate schedule."""
+    import ctypes as C
+    from quantummatter_amd._lib import lib, ptr
+    from quantummatter_amd.backend import (SCHED_COMPILED, SCHED_PER_GATE, HipBackendOptions,
+                                           HipStateVectorBackend)
+    n = 26
+    ops = workloads.random_layer_circuit(n, 40, seed=2).to_ops()
+    four = C.c_int32()
+    assert lib().qm_jit_compile_check(n, 0, ptr(ops), len(ops), 14, 9, C.byref(four)) == 0
+    with HipStateVectorBackend(HipBackendOptions(EnableProfiling=True)) as be:
+        with be.CreateSession(n, C128) as a, be.CreateSession(n, C128) as b:
+            a.EvolveOps(ops, SCHED_PER_GATE)
+            p = b.EvolveOps(ops, SCHED_COMPILED, profile=True)
+            assert p.Passes <= four.value
+            assert abs(1 - abs(b.Inner(a))) <= 1e-12
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,layers,seed", [(25, 40, 2), (28, 40, 1), (29, 12, 1)])
 def test_compiled_schedule_wide_layer_circuits(n, layers, seed):
