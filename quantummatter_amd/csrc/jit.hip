@@ -217,8 +217,9 @@ std::string hex(uint64_t v) {
 }
 
 // Source of one pass kernel, and the coefficient layout it reads.
-// Code-generation variant (tuning knob QM_JIT_VARIANT; default 401 = bits 0,
-// 4, 7 and 8 in complex128, 913 = those and bit 9 in complex64): bit 0 exchanges through an LDS buffer of reals (half the LDS,
+// Code-generation variant (tuning knob QM_JIT_VARIANT; default 1425 = bits 0,
+// 4, 7, 8 and 10 in complex128, 913 = bits 0, 4, 7, 8 and 9 in complex64):
+// bit 0 exchanges through an LDS buffer of reals (half the LDS,
 // real then imaginary parts, so more workgroups fit a CU), bit 2 forms the
 // lane part of each LDS address at its transpose instead of once per kernel,
 // bit 3 loads and stores the state with non-temporal hints (a pass never
@@ -240,13 +241,17 @@ std::string hex(uint64_t v) {
 // each swizzled LDS address into the XOR of the few distinct parts that
 // overlap the lane part and an immediate offset (fewer address instructions
 // and paired LDS accesses; 6 % faster in complex64, 3 % slower in complex128
-// at 16384-amplitude tiles, where the extra live addresses spill).
+// at 16384-amplitude tiles, where the extra live addresses spill), bit 10
+// (with bit 0) lays out each exchange in LDS so that the lanes one LDS
+// instruction serves hit distinct banks on both sides (exchange_layout; the
+// XOR-fold layout of the tables left 76 % extra bank-conflict cycles; 4 %
+// faster in complex128, 6 % slower than bit 9 in complex64).
 int jit_variant(int precision) {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
     return e ? std::atoi(e) : -1;
   }();
-  return v >= 0 ? v : precision == QM_C64 ? 913 : 401;
+  return v >= 0 ? v : precision == QM_C64 ? 913 : 1425;
 }
 
 int variant_of(const FusedShape &sh, int precision) {
@@ -258,6 +263,114 @@ size_t jit_lds_bytes(int precision, const FusedShape &sh) {
   return (split ? (size_t)amp_bytes(precision) / 2 : (size_t)amp_bytes(precision)) << sh.K;
 }
 
+// Rank of vectors over GF(2) (bit masks).
+static int gf2_rank(std::vector<uint32_t> v) {
+  int rank = 0;
+  for (int bit = 31; bit >= 0; bit--) {
+    size_t i = rank;
+    while (i < v.size() && !((v[i] >> bit) & 1)) i++;
+    if (i == v.size()) continue;
+    std::swap(v[rank], v[i]);
+    for (size_t k = 0; k < v.size(); k++)
+      if (k != (size_t)rank && ((v[k] >> bit) & 1)) v[k] ^= v[rank];
+    rank++;
+  }
+  return rank;
+}
+
+// LDS layout of one exchange: an invertible GF(2)-linear map S on the K tile
+// bits, given by the images cols[p] of the unit vectors, under which the
+// amplitude at tile position x sits at element S(x).  Writers (mapping
+// `from`) and readers (mapping `to`) both address through S, so any
+// invertible S is correct; it is chosen so that the lanes an LDS instruction
+// serves together hit distinct banks on both sides: for the first b thread
+// bits (b lanes per 128 bytes, and b + 1 per 256 bytes), the low b (b + 1)
+// bits of their images are independent.  S keeps bits >= B (B = b + 1) in
+// place and gives every position a B-bit low part: positions below B a basis
+// of the low bits, the others any value (block triangular, so invertible).
+// Values are found by a seeded randomised search; identity if none is found.
+static std::vector<uint32_t> exchange_layout(const DevPass &d, int from, int to, int K, int L,
+                                             int elem_bytes) {
+  int b = 0;
+  while ((elem_bytes << (b + 1)) <= 128) b++;
+  b = std::min({b, L - 1, K - 1});
+  const int B = b + 1;
+  std::vector<uint32_t> cols(K);
+  for (int p = 0; p < K; p++) cols[p] = 1u << p;
+  if (b < 1) return cols;
+  const uint32_t low = (1u << B) - 1;
+  // Groups: (positions, bits that must be independent).
+  std::vector<std::pair<std::vector<int>, int>> groups;
+  for (int slot : {from, to})
+    for (int n : {b, B}) {
+      std::vector<int> g;
+      for (int k = 0; k < n; k++) g.push_back(d.lanepos[slot][k]);
+      groups.push_back({g, n});
+    }
+  std::vector<int> order;
+  for (auto &g : groups)
+    for (int p : g.first)
+      if (std::find(order.begin(), order.end(), p) == order.end()) order.push_back(p);
+  uint64_t rng = 0x9E3779B97F4A7C15ull ^ ((uint64_t)from << 32) ^ (uint64_t)to;
+  auto next = [&] {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
+  std::vector<uint32_t> val(K, 0);
+  std::vector<char> set(K, 0);
+  auto ok = [&] {
+    for (auto &g : groups) {
+      std::vector<uint32_t> v;
+      for (int p : g.first)
+        if (set[p]) v.push_back(val[p] & ((1u << g.second) - 1));
+      if (gf2_rank(v) != (int)v.size()) return false;
+    }
+    std::vector<uint32_t> basis;  // low parts of positions below B stay independent
+    for (int p = 0; p < B && p < K; p++)
+      if (set[p]) basis.push_back(val[p]);
+    return gf2_rank(basis) == (int)basis.size();
+  };
+  for (int attempt = 0; attempt < 4000; attempt++) {
+    std::fill(set.begin(), set.end(), 0);
+    bool dead = false;
+    for (int p : order) {
+      set[p] = 1;
+      bool found = false;
+      const uint32_t start = (uint32_t)next();
+      for (uint32_t c = 0; c < low && !found; c++) {
+        val[p] = ((start + c) % low) + 1;
+        found = ok();
+      }
+      if (!found) {
+        dead = true;
+        break;
+      }
+    }
+    if (dead) continue;
+    // Complete the low basis with the unused positions below B.
+    for (int p = 0; p < B && p < K; p++) {
+      if (set[p]) continue;
+      set[p] = 1;
+      bool found = false;
+      for (uint32_t c = 1; c <= low && !found; c++) {
+        val[p] = c;
+        found = ok();
+      }
+      if (!found) {
+        dead = true;
+        break;
+      }
+    }
+    if (dead) continue;
+    for (int p = 0; p < K; p++) cols[p] = (p < B ? 0u : (1u << p)) | (set[p] ? val[p] : 0u);
+    return cols;
+  }
+  for (int p = 0; p < K; p++) cols[p] = 1u << p;
+  return cols;
+}
+
 std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
                         std::vector<double> &coef) {
   const int variant = variant_of(sh, precision);
@@ -265,6 +378,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const bool nontemporal = (variant & 8) != 0, factored = (variant & 16) != 0, frames = (variant & 32) != 0;
   const bool prefetch = (variant & 64) != 0, tile_coef = (variant & 128) != 0;
   const bool launder = (variant & 256) != 0, fold = (variant & 512) != 0;
+  const bool banked = (variant & 1024) != 0 && split;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -368,6 +482,37 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     // addresses are computed there instead of being hoisted out of the tile
     // loop and held, for every exchange of the pass at once, in (spilled)
     // registers.
+    if (banked) {
+      // Per-exchange layout (exchange_layout): lane parts formed here from a
+      // laundered thread index, register parts as constants.
+      const std::vector<uint32_t> cols =
+          exchange_layout(d, from, to, sh.K, L, c128 ? 8 : 4);
+      auto S = [&](uint32_t x) {
+        uint32_t y = 0;
+        for (int q = 0; q < sh.K; q++)
+          if ((x >> q) & 1) y ^= cols[q];
+        return y;
+      };
+      auto lane_expr = [&](int slot) {
+        std::string e = "(0u";
+        for (int k = 0; k < L; k++)
+          e += " ^ ((tt & " + std::to_string(1u << k) + "u) ? " + std::to_string(S(1u << d.lanepos[slot][k])) +
+               "u : 0u)";
+        return e + ")";
+      };
+      o << "    unsigned tt = t;\n    asm volatile(\"\" : \"+v\"(tt));\n";
+      o << "    const unsigned jw = " << lane_expr(from) << ", jr = " << lane_expr(to) << ";\n";
+      for (const char *part : {"vr", "vi"}) {
+        o << "    __syncthreads();\n";
+        for (int r = 0; r < N; r++)
+          o << "    lds[jw ^ " << S(d.regloc[from][r]) << "u] = " << part << "[" << r << "];\n";
+        o << "    __syncthreads();\n";
+        for (int r = 0; r < N; r++)
+          o << "    " << part << "[" << r << "] = lds[jr ^ " << S(d.regloc[to][r]) << "u];\n";
+      }
+      o << "    }\n";
+      return;
+    }
     const std::string jw = launder ? "jw" : "j" + std::to_string(from);
     const std::string jr = launder ? "jr" : "j" + std::to_string(to);
     // Address of register r under a mapping: the lane part j XOR the
