@@ -192,9 +192,10 @@ struct FusedShape {
   // with it the registers its generated kernel needs.
   int max_phases = 0;
   // Compiled schedule only (0: off): the circuit is also planned with this
-  // many low bits kept in the tiles, and the plan with fewer passes (then
-  // fewer transposes) runs.  The greedy tile search finds fewer passes with
-  // one or the other value circuit by circuit.
+  // many low bits kept in the tiles, each with at most 9 phases per pass as
+  // well, and the plan with the lowest modelled time runs (see run_gates).
+  // The greedy tile search finds fewer passes with one or the other value
+  // circuit by circuit.
   int alt_contiguous = 0;
 };
 

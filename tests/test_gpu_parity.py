@@ -341,9 +341,10 @@ def test_dense_blocks_layer_circuit_16q():
 
 def test_compiled_schedule_plans_with_both_low_bit_counts():
     """The complex128 compiled schedule plans with 3 and with 4 low bits kept
-    in the tiles and runs the plan with fewer passes: never more passes than
-    the 4-bit plan alone (qm_jit_compile_check plans with 4), and the same
-    state as the per-gate schedule."""
+    in the tiles (full and 9-phase budgets) and runs the plan of lowest
+    modelled time; on this circuit that is the 4-bit plan (12 passes against
+    13 with 3 bits; qm_jit_compile_check plans with 4), and the state matches
+    the per-gate schedule."""
     import ctypes as C
     from quantummatter_amd._lib import lib, ptr
     from quantummatter_amd.backend import (SCHED_COMPILED, SCHED_PER_GATE, HipBackendOptions,
