@@ -218,7 +218,7 @@ std::string hex(uint64_t v) {
 
 // Source of one pass kernel, and the coefficient layout it reads.
 // Code-generation variant (tuning knob QM_JIT_VARIANT; default 1425 = bits 0,
-// 4, 7, 8 and 10 in complex128, 913 = bits 0, 4, 7, 8 and 9 in complex64):
+// 4, 7, 8 and 10 in complex128, 1937 = those and bit 9 in complex64):
 // bit 0 exchanges through an LDS buffer of reals (half the LDS,
 // real then imaginary parts, so more workgroups fit a CU), bit 2 forms the
 // lane part of each LDS address at its transpose instead of once per kernel,
@@ -245,13 +245,13 @@ std::string hex(uint64_t v) {
 // (with bit 0) lays out each exchange in LDS so that the lanes one LDS
 // instruction serves hit distinct banks on both sides (exchange_layout; the
 // XOR-fold layout of the tables left 76 % extra bank-conflict cycles; 4 %
-// faster in complex128, 6 % slower than bit 9 in complex64).
+// faster in complex128; with bit 9, 4 % faster than bit 9 alone in complex64).
 int jit_variant(int precision) {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
     return e ? std::atoi(e) : -1;
   }();
-  return v >= 0 ? v : precision == QM_C64 ? 913 : 1425;
+  return v >= 0 ? v : precision == QM_C64 ? 1937 : 1425;
 }
 
 int variant_of(const FusedShape &sh, int precision) {
@@ -500,15 +500,23 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
                "u : 0u)";
         return e + ")";
       };
+      // With fold, the register constant's bits outside every lane image are
+      // added (immediate offset) instead of XORed, as in `at` below.
+      auto addr = [&](const char *j, int slot, uint32_t c) {
+        if (!fold) return std::string("lds[") + j + " ^ " + std::to_string(c) + "u]";
+        uint32_t m = 0;
+        for (int k = 0; k < L; k++) m |= S(1u << d.lanepos[slot][k]);
+        return std::string("lds[(") + j + " ^ " + std::to_string(c & m) + "u) + " + std::to_string(c & ~m) + "u]";
+      };
       o << "    unsigned tt = t;\n    asm volatile(\"\" : \"+v\"(tt));\n";
       o << "    const unsigned jw = " << lane_expr(from) << ", jr = " << lane_expr(to) << ";\n";
       for (const char *part : {"vr", "vi"}) {
         o << "    __syncthreads();\n";
         for (int r = 0; r < N; r++)
-          o << "    lds[jw ^ " << S(d.regloc[from][r]) << "u] = " << part << "[" << r << "];\n";
+          o << "    " << addr("jw", from, S(d.regloc[from][r])) << " = " << part << "[" << r << "];\n";
         o << "    __syncthreads();\n";
         for (int r = 0; r < N; r++)
-          o << "    " << part << "[" << r << "] = lds[jr ^ " << S(d.regloc[to][r]) << "u];\n";
+          o << "    " << part << "[" << r << "] = " << addr("jr", to, S(d.regloc[to][r])) << ";\n";
       }
       o << "    }\n";
       return;

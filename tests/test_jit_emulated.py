@@ -185,7 +185,7 @@ def _variant_case(variant, kind, n, seed, tmp_path):
                                                  (129, "layer", 16, 8), (401, "all", 15, 10),
                                                  (401, "layer", 16, 11), (913, "all", 15, 12),
                                                  (913, "layer", 16, 13), (1425, "all", 15, 14),
-                                                 (1425, "layer", 16, 15)])
+                                                 (1425, "layer", 16, 15), (1937, "all", 15, 16)])
 def test_generated_kernel_variants_match_oracle(tmp_path, variant, kind, n, seed):
     """Optional code variants: factored forms (bit 4), exchange flags (bit 5),
     per-tile coefficient loads (bit 7), per-exchange lane addresses (bit 8) and
