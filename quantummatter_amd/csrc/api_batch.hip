@@ -573,8 +573,11 @@ void check_parameters(const double *parameters, int64_t n_params) {
 
 // The 1 + 2R items of a parameter-shift gradient: item 0 the bound circuit,
 // items 1 + 2r / 2 + 2r rotation occurrence r shifted by +pi/2 / -pi/2
-// (VariationalEnergyEvaluator.cs:50-80).  The shift is folded into the
-// rotation's angle: R(a) R(s) = R(a + s) for one axis.
+// (VariationalEnergyEvaluator.cs:50-80).  The reference appends a second
+// same-axis rotation of angle +-pi/2 (ParameterizedCircuit.cs:49-51,85-89) so
+// that the shift survives a huge angle; here the two prepared rotations are
+// multiplied on the host into one (see shifted_angle), which keeps the shift
+// for every finite angle.
 BatchJob gradient_job(int qubits, int precision, const Template &tp, const qm_pauli *terms,
                       int64_t n_terms) {
   const int64_t R = (int64_t)tp.rot_kind.size();
@@ -591,13 +594,30 @@ BatchJob gradient_job(int qubits, int precision, const Template &tp, const qm_pa
   return job;
 }
 
+// Coefficients of R(shift) R(a) for one axis, shift = +-pi/2.  Rx, Ry and Rz
+// prepared at angle a are all determined by (c, s) = (cos, sin)(a/2)
+// (PreparedStateVectorGate.cs:26-45), and the product of two of them is the
+// rotation with (c cb - s sb, s cb + c sb), (cb, sb) = (cos, sin)(shift/2):
+// the shifted rotation is formed from the prepared coefficients of both
+// gates, never from a + shift, which rounds to a once |a| exceeds ~2^53 pi/2
+// (a = 1e100 + pi/2 == 1e100) and loses ~|a| 2^-53 of the shift before that.
+void shifted_angle(double *dst, double a, double shift) {
+  const double c = std::cos(a / 2), s = std::sin(a / 2);
+  const double cb = std::cos(shift / 2), sb = std::sin(shift / 2);
+  dst[0] = c * cb - s * sb;
+  dst[1] = s * cb + c * sb;
+}
+
 void gradient_angles(const Template &tp, const double *parameters, int32_t slots, double *angles) {
   const int64_t R = (int64_t)tp.rot_kind.size();
   for (int64_t i = 0; i < 1 + 2 * R; i++)
     for (int64_t r = 0; r < R; r++) {
-      double a = parameters[tp.rot_param[(size_t)r]];
-      if (i > 0 && (i - 1) / 2 == r) a += (i % 2 == 1) ? M_PI / 2 : -M_PI / 2;
-      put_angle(&angles[(size_t)((i * slots + r) * 2)], a);
+      const double a = parameters[tp.rot_param[(size_t)r]];
+      double *dst = &angles[(size_t)((i * slots + r) * 2)];
+      if (i > 0 && (i - 1) / 2 == r)
+        shifted_angle(dst, a, (i % 2 == 1) ? M_PI / 2 : -M_PI / 2);
+      else
+        put_angle(dst, a);
     }
 }
 
@@ -676,8 +696,11 @@ struct GraphedBatch {
   size_t off_numer = 0, off_mass = 0, off_fb = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int64_t held = 0;        // arena bytes counted in the context's resident bytes
+  uint64_t last_use = 0;   // program tick of the last evaluation (cache eviction)
 
   ~GraphedBatch() {
+    if (c && held) c->resident_bytes -= held;
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (h_angles) (void)hipHostFree(h_angles);
@@ -707,6 +730,10 @@ struct GraphedBatch {
     job = std::move(j);
     g = build_launch(job, L);
     A.setup(c, L);
+    // The arena stays allocated while the program lives: later state and batch
+    // budget checks see it.
+    held = (int64_t)A.ar.size;
+    c->resident_bytes += held;
     const int64_t n = (int64_t)L.items.size();
     if (hipHostMalloc((void **)&h_angles, std::max<size_t>(L.angles.size(), 1) * 8) != hipSuccess ||
         (off_numer = ((size_t)n * 4 + 7) & ~(size_t)7, off_mass = off_numer + (size_t)L.n_numer * 8,
@@ -756,7 +783,12 @@ struct VqeProgram {
   std::vector<double> coeffs;
   std::unique_ptr<GraphedBatch> grad;
   std::map<int64_t, std::unique_ptr<GraphedBatch>> energy;  // by number of parameter sets
+  uint64_t tick = 0;
 };
+
+// Graphed energy batches kept per program (distinct n_sets values); the least
+// recently used one is freed to make room for another.
+constexpr size_t kMaxEnergyGraphs = 4;
 
 Ctx *batch_ctx(qm_ctx *h, qm_profile *profile) {
   Ctx *c = reinterpret_cast<Ctx *>(h);
@@ -961,6 +993,13 @@ int qm_vqe_program_energies(qm_vqe_program *h, const double *parameter_sets, int
     if (!energies) fail(QM_ERR_ARG, "energies is null");
     std::lock_guard<std::recursive_mutex> lk(p->c->lock);
     const int64_t n_terms = (int64_t)p->terms.size();
+    if (!p->energy.count(n_sets) && p->energy.size() >= kMaxEnergyGraphs) {
+      auto lru = p->energy.begin();
+      for (auto it = p->energy.begin(); it != p->energy.end(); ++it)
+        if (it->second->last_use < lru->second->last_use) lru = it;
+      QM_HIP(hipStreamSynchronize(p->c->stream));
+      p->energy.erase(lru);
+    }
     std::unique_ptr<GraphedBatch> &slot = p->energy[n_sets];
     if (!slot) {
       auto gb = std::make_unique<GraphedBatch>();
@@ -969,6 +1008,7 @@ int qm_vqe_program_energies(qm_vqe_program *h, const double *parameter_sets, int
       gb->build(p->c, std::move(job));
       slot = std::move(gb);
     }
+    slot->last_use = ++p->tick;
     energy_angles(p->tp, parameter_sets, p->n_params, n_sets, slot->job.slots, slot->h_angles);
     BatchOut out;
     slot->replay(profile != nullptr, out);

@@ -496,11 +496,18 @@ int qm_server_submit(qm_server *h, const qm_request *rq, int64_t *id) {
     sv->submitted++;
     Req *raw = r.get();
     sv->reqs[raw->id] = std::move(r);
+    *id = raw->id;
+    if (raw->bytes > sv->budget) {
+      // Can never be admitted, whichever lane would take it: fails now (its
+      // wait reports the budget error) instead of waiting in a queue forever.
+      finish(sv, raw, QM_ERR_BUDGET, "Request exceeds the device memory budget.");
+      sv->cv_done.notify_all();
+      return;
+    }
     if (raw->qubits <= sv->batch_max_qubits)
       sv->small_q.push_back(raw);
     else
       sv->large_q.push_back(raw);
-    *id = raw->id;
     sv->cv_work.notify_all();
   });
 }

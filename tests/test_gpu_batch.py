@@ -5,9 +5,10 @@ and shot uniforms, every terminal sample, MeasureAll outcome and pool draw
 count equals the oracle's (complex128).  Variational batches
 (qm_energy_gradient_batch, qm_energy_batch): energy and parameter-shift
 gradient within 1e-11 relative of the oracle in complex128 (the shifted
-rotation is folded into one rotation, R(a)R(+-pi/2) = R(a +- pi/2), so the
-amplitudes differ from the reference's two-gate form by rounding only), and
-within 1e-4 of the coefficient norm in complex64 (FP64 reductions).
+rotation is the host product of the two prepared rotations, so the amplitudes
+differ from the reference's two-gate form by rounding only, at any finite
+angle), and within 1e-4 of the coefficient norm in complex64 (FP64
+reductions).
 """
 import math
 
@@ -21,7 +22,7 @@ from quantummatter_amd import (NoiseChannel, NoisyCircuit, ParameterizedCircuit,
                                workloads)
 from quantummatter_amd.backend import C64, C128
 from quantummatter_amd.batched import (HipBatchedEnergyEvaluator, HipTrajectoryRunner,
-                                       batch_min_qubits)
+                                       HipVqeProgram, batch_min_qubits)
 
 pytestmark = pytest.mark.gpu
 
@@ -157,3 +158,59 @@ def test_shared_ry_parameter_shift_closed_form():
         res = HipBatchedEnergyEvaluator(shared_backend()).EvaluateWithGradient(pc, ham, [theta])
         assert abs(res.Energy - math.cos(2 * theta)) <= 1e-12
         assert abs(res.Gradient[0] + 2 * math.sin(2 * theta)) <= 1e-12
+
+
+def _near(expected, actual):
+    """VariationalEnergyTests.cs:293: |expected - actual| <= 5e-12."""
+    assert abs(expected - actual) <= 5e-12, (expected, actual)
+
+
+def _single_ry(n):
+    return ParameterizedCircuit(n, 1).Ry(0, 0)
+
+
+def _z(n, coefficient=1.0):
+    return PauliHamiltonian(n, [PauliTerm(coefficient, PauliString("I" * (n - 1) + "Z"))])
+
+
+def _gradient_through(path, circuit, ham, theta):
+    if path == "batch":
+        return HipBatchedEnergyEvaluator(shared_backend()).EvaluateWithGradient(circuit, ham, theta)
+    with HipVqeProgram(shared_backend(), circuit, ham) as program:
+        return program.EvaluateWithGradient(circuit, ham, theta)
+
+
+@pytest.mark.parametrize("path", ["batch", "program"])
+def test_large_angle_parameter_shift_cases(path):
+    """VariationalEnergyTests.cs:24-44 through qm_energy_gradient_batch and
+    qm_vqe_program_gradient, widened to the batch minimum width with idle
+    qubits: theta = 1e100 (where 1e100 + pi/2 == 1e100, so a shift added to the
+    angle is lost), a 1e16 identity offset and a 1e308 coefficient."""
+    n = batch_min_qubits(C128)
+    huge = 1e100
+    assert huge + math.pi / 2 == huge
+    res = _gradient_through(path, _single_ry(n), _z(n), [huge])
+    _near(math.cos(huge), res.Energy)
+    _near(-math.sin(huge), res.Gradient[0])
+    angle = 0.37
+    offset = PauliHamiltonian(n, [PauliTerm(1e16, PauliString("I" * n)),
+                                  PauliTerm(1.0, PauliString("I" * (n - 1) + "Z"))])
+    _near(-math.sin(angle), _gradient_through(path, _single_ry(n), offset, [angle]).Gradient[0])
+    large = _gradient_through(path, _single_ry(n), _z(n, 1e308), [math.pi / 2])
+    assert abs(large.Gradient[0] / 1e308 + 1) <= 4e-12
+
+
+@pytest.mark.parametrize("path", ["batch", "program"])
+def test_hea_gradient_at_large_angles_matches_oracle(path):
+    """A hardware-efficient ansatz with every angle near 1e6: the oracle
+    appends the shift as a second rotation; the batched shift must agree to
+    the 1e-11 relative bar (an angle-added shift loses ~1e6 * 2^-53 of it)."""
+    n = 11
+    circuit = workloads.hardware_efficient_ansatz(n, 2)
+    theta = 1e6 + workloads.random_parameters(circuit.ParameterCount, 77)
+    ham = workloads.random_hamiltonian(n, 48, 78)
+    e_want, g_want = oracle_gradient(circuit, theta, ham)
+    res = _gradient_through(path, circuit, ham, theta)
+    scale = sum(abs(t.Coefficient) for t in ham.Terms)
+    assert abs(res.Energy - e_want) <= 1e-11 * scale
+    assert np.max(np.abs(np.array(res.Gradient) - g_want)) <= 1e-11 * scale

@@ -78,3 +78,29 @@ def test_program_errors():
     narrow, nham = problem(4)
     with pytest.raises(NotImplementedError):
         HipVqeProgram(be, narrow, nham)
+
+
+def test_program_arenas_are_counted_and_bounded():
+    """A program's graphed arenas count in the context's resident bytes while
+    it lives, at most four energy batch sizes stay cached (the least recently
+    used is freed first), and destroying the program returns the bytes."""
+    from quantummatter_amd.backend import HipStateVectorBackend
+    n = 11
+    circuit, ham = problem(n, seed=9)
+    with HipStateVectorBackend() as be:
+        base = be.ResidentBytes()
+        with HipVqeProgram(be, circuit, ham) as prog:
+            theta = workloads.random_parameters(circuit.ParameterCount, seed=3)
+            prog.EvaluateWithGradient(circuit, ham, theta)
+            assert be.ResidentBytes() > base
+            for sets in range(1, 8):
+                prog.EvaluateMany(circuit, ham, np.stack([theta] * sets))
+            # Sizes 4..7 are cached: evaluating them again allocates nothing.
+            held = be.ResidentBytes()
+            for sets in (4, 5, 6, 7):
+                prog.EvaluateMany(circuit, ham, np.stack([theta] * sets))
+                assert be.ResidentBytes() == held
+            # Size 1 was evicted: it is rebuilt in place of the oldest (4).
+            prog.EvaluateMany(circuit, ham, np.stack([theta]))
+            assert be.ResidentBytes() < held
+        assert be.ResidentBytes() == base

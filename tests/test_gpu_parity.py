@@ -374,3 +374,29 @@ def test_compiled_schedule_wide_layer_circuits(n, layers, seed):
         b.EvolveOps(ops, SCHED_COMPILED)
         assert abs(1 - abs(b.Inner(a))) <= 1e-12
         assert abs(b.Mass() - 1) <= 1e-12
+
+
+def test_known_wrong_code_variant_is_refused_on_the_device():
+    """Factored forms without per-tile coefficient loads (QM_JIT_VARIANT=17)
+    computed wrong device states for some passes; the library refuses to load
+    such a variant (NotSupported) before compiling anything, while the default
+    runs.  Checked in a child process: the variant is read once per process."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "from quantummatter_amd import workloads\n"
+        "from quantummatter_amd.backend import HipStateVectorBackend, SCHED_COMPILED\n"
+        "c = workloads.random_layer_circuit(16, 4, seed=3)\n"
+        "with HipStateVectorBackend() as be, be.CreateSession(16) as s:\n"
+        "    try:\n"
+        "        s.Evolve(c, SCHED_COMPILED)\n"
+        "        print('ran')\n"
+        "    except NotImplementedError as e:\n"
+        "        print('refused', e)\n")
+    for variant, want in (("17", "refused"), ("1425", "ran")):
+        env = dict(os.environ, QM_JIT_VARIANT=variant, PYTHONPATH=root)
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        assert r.stdout.split()[0] == want, r.stdout

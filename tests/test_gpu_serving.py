@@ -96,3 +96,15 @@ def test_admission_respects_budget_and_rejects_oversize():
         big = sv.Submit(workloads.random_layer_circuit(22, 1, seed=1), AMPLITUDES, C128)
         with pytest.raises(InvalidOperationError):
             sv.Wait(big)
+
+
+def test_small_request_over_a_tiny_budget_fails_instead_of_waiting():
+    """A batched-width request whose state exceeds the whole budget is failed
+    at submission (its wait reports the budget error), so wait, drain and
+    dispose return."""
+    with HipRequestServer(full_lanes=1, batch_max_qubits=14, device_budget_bytes=4096) as sv:
+        rid = sv.Submit(workloads.random_layer_circuit(10, 2, seed=1), AMPLITUDES, C128)
+        with pytest.raises(InvalidOperationError):
+            sv.Wait(rid)
+        sv.Drain()
+        assert sv.Stats().failed == 1
