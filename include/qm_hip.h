@@ -224,6 +224,21 @@ int qm_state_expectations(qm_state *state, const qm_pauli *terms, int64_t n_term
 /* StateVector.ExpectationEnergy (src/QuantumMatter.Core/StateVector.Energy.cs:10-46). */
 int qm_state_energy(qm_state *state, const qm_pauli *terms, const double *coefficients,
                     int64_t n_terms, double *energy);
+/* No counterpart in the reference: qm_state_energy for callers that cannot
+ * allocate (code between hipStreamBeginCapture/EndCapture elsewhere on the
+ * device, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, checks and result bit for bit, but it allocates nothing: its
+ * partial sums, term tables and results live in a static device array of
+ * libqmhip.so (about 17 MB per device, held from the first context on that
+ * device until the process ends), its host tables on the stack, and its
+ * copies use the caller's memory.  At most QM_NOALLOC_MAX_TERMS terms with
+ * nonzero coefficients (QM_ERR_RANGE beyond).  All calls of a process on one
+ * device share that device's array and would race on it if they overlapped:
+ * the library serialises them with a per-device lock held until the call's
+ * last copy completes. */
+#define QM_NOALLOC_MAX_TERMS 512
+int qm_state_energy_noalloc(qm_state *state, const qm_pauli *terms, const double *coefficients,
+                            int64_t n_terms, double *energy);
 /* <a|b> over two states of equal width, FP64 accumulation (re, im). */
 int qm_state_inner(qm_state *a, qm_state *b, double *re, double *im);
 

@@ -69,6 +69,7 @@ SIGNATURES = {
     "qm_state_sample": [P, P, I64, P],
     "qm_state_expectations": [P, P, I64, P],
     "qm_state_energy": [P, P, P, I64, P],
+    "qm_state_energy_noalloc": [P, P, P, I64, P],
     "qm_state_inner": [P, P, P, P],
     "qm_state_boundary_fallbacks": [P, P],
     "qm_plan_fused": [I32, I32, P, I64, P, P, P],

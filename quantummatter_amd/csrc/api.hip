@@ -330,6 +330,7 @@ int qm_ctx_create(const qm_ctx_options *opt, qm_ctx **out) {
       QM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       QM_HIP(hipEventCreate(&c->ev_start));
       QM_HIP(hipEventCreate(&c->ev_stop));
+      noalloc_setup(c->device);  // the scratch array's address on this device (once per device)
     } catch (...) {
       delete c;
       throw;
@@ -707,17 +708,42 @@ int qm_state_expectations(qm_state *h, const qm_pauli *terms, int64_t n, double 
   });
 }
 
+// Validation of qm_state_energy's arguments (StateVector.Energy.cs:10-25).
+static void check_energy_args(const State *s, const qm_pauli *terms, const double *coeffs, int64_t n,
+                              const double *energy) {
+  if (!energy || n < 0 || (n > 0 && (!terms || !coeffs))) fail(QM_ERR_ARG, "buffers are null");
+  const uint64_t all = s->qubits >= 64 ? ~0ull : ((1ull << s->qubits) - 1);
+  for (int64_t i = 0; i < n; i++) {
+    if ((terms[i].flip_mask | terms[i].sign_mask) & ~all)
+      fail(QM_ERR_ARG, "Hamiltonian and state sizes must match.");
+    if (!std::isfinite(coeffs[i])) fail(QM_ERR_RANGE, "coefficient");
+  }
+}
+
+// Neumaier summation in term order (StateVector.Energy.cs:26-45) of
+// coeffs[idx[k]] * num[k] / total over the terms with nonzero coefficients.
+static double neumaier_energy(const double *num, const int64_t *idx, int64_t n_used, const double *coeffs,
+                              double total) {
+  double sum = 0, corr = 0;
+  for (int64_t k = 0; k < n_used; k++) {
+    if (!std::isfinite(num[k])) fail(QM_ERR_NONFINITE, "The observable accumulation is nonfinite.");
+    const double value = coeffs[idx[k]] * (num[k] / total);
+    const double next = sum + value;
+    corr += std::fabs(sum) >= std::fabs(value) ? (sum - next) + value : (value - next) + sum;
+    sum = next;
+    if (!std::isfinite(sum) || !std::isfinite(corr))
+      fail(QM_ERR_NONFINITE, "The energy accumulation is nonfinite.");
+  }
+  const double e = sum + corr;
+  if (!std::isfinite(e)) fail(QM_ERR_NONFINITE, "The energy is nonfinite.");
+  return e;
+}
+
 int qm_state_energy(qm_state *h, const qm_pauli *terms, const double *coeffs, int64_t n,
                     double *energy) {
   return guarded([&] {
     State *s = as_state(h);
-    if (!energy || n < 0 || (n > 0 && (!terms || !coeffs))) fail(QM_ERR_ARG, "buffers are null");
-    const uint64_t all = s->qubits >= 64 ? ~0ull : ((1ull << s->qubits) - 1);
-    for (int64_t i = 0; i < n; i++) {
-      if ((terms[i].flip_mask | terms[i].sign_mask) & ~all)
-        fail(QM_ERR_ARG, "Hamiltonian and state sizes must match.");
-      if (!std::isfinite(coeffs[i])) fail(QM_ERR_RANGE, "coefficient");
-    }
+    check_energy_args(s, terms, coeffs, n, energy);
     std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
     const double total = reduce_mass(s, s->ctx->stream).total;
     validate_mass(total);
@@ -730,20 +756,34 @@ int qm_state_energy(qm_state *h, const qm_pauli *terms, const double *coeffs, in
       }
     std::vector<double> num(used.size());
     reduce_pauli(s, used.data(), (int64_t)used.size(), num.data(), s->ctx->stream);
-    // Neumaier summation in term order (StateVector.Energy.cs:26-45).
-    double sum = 0, corr = 0;
-    for (size_t k = 0; k < used.size(); k++) {
-      if (!std::isfinite(num[k])) fail(QM_ERR_NONFINITE, "The observable accumulation is nonfinite.");
-      const double value = coeffs[idx[k]] * (num[k] / total);
-      const double next = sum + value;
-      corr += std::fabs(sum) >= std::fabs(value) ? (sum - next) + value : (value - next) + sum;
-      sum = next;
-      if (!std::isfinite(sum) || !std::isfinite(corr))
-        fail(QM_ERR_NONFINITE, "The energy accumulation is nonfinite.");
-    }
-    const double e = sum + corr;
-    if (!std::isfinite(e)) fail(QM_ERR_NONFINITE, "The energy is nonfinite.");
-    *energy = e;
+    *energy = neumaier_energy(num.data(), idx.data(), (int64_t)used.size(), coeffs, total);
+  });
+}
+
+// The same energy with every device buffer in the library's static scratch
+// array and every host table on the stack: no allocation on any path that
+// succeeds (the terms with nonzero coefficients are limited to
+// QM_NOALLOC_MAX_TERMS).
+int qm_state_energy_noalloc(qm_state *h, const qm_pauli *terms, const double *coeffs, int64_t n,
+                            double *energy) {
+  return guarded([&] {
+    State *s = as_state(h);
+    check_energy_args(s, terms, coeffs, n, energy);
+    qm_pauli used[QM_NOALLOC_MAX_TERMS];
+    int64_t idx[QM_NOALLOC_MAX_TERMS];
+    int64_t n_used = 0;
+    for (int64_t i = 0; i < n; i++)
+      if (coeffs[i] != 0) {
+        if (n_used == QM_NOALLOC_MAX_TERMS)
+          fail(QM_ERR_RANGE, "more Hamiltonian terms than the scratch buffer holds (QM_NOALLOC_MAX_TERMS)");
+        used[n_used] = terms[i];
+        idx[n_used++] = i;
+      }
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    double total = 0, num[QM_NOALLOC_MAX_TERMS];
+    reduce_energy_noalloc(s, used, n_used, &total, num, s->ctx->stream);
+    validate_mass(total);
+    *energy = neumaier_energy(num, idx, n_used, coeffs, total);
   });
 }
 

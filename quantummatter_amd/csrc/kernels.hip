@@ -9,6 +9,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "dev_types.hpp"
@@ -822,6 +825,118 @@ int64_t cdf_search(State *s, const double *uniforms, int64_t shots, int64_t *out
   QM_HIP(hipStreamSynchronize(st));
   if (total_out) *total_out = pin[0];
   return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// Energy reductions without allocation (qm_state_energy_noalloc).
+//
+// The same kernels, block counts and fold order as reduce_mass /
+// reduce_pauli, so the mass and every numerator are bit-identical to theirs,
+// but every device buffer they need lives in one file-scope array of this
+// library's code object: mass partials [blocks] | term partials
+// [terms x blocks] | results [1 + terms] | term groups.  Host tables are on
+// the stack, copies go to and from the caller's pageable memory.  Each device
+// holds its own copy of the array, at its own address, from the first call of
+// noalloc_setup on that device until the process ends.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kNaBlocks = 4096;  // reduction_blocks' maximum
+constexpr int64_t kNaTerms = QM_NOALLOC_MAX_TERMS;
+constexpr int64_t kNaGroupDoubles = (int64_t)((sizeof(PauliGroup) + 7) / 8);
+constexpr int64_t kNaDoubles = kNaBlocks * (1 + kNaTerms) + (1 + kNaTerms) + kNaTerms * kNaGroupDoubles;
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+constexpr int kNaDevices = 64;
+struct NaSlot {
+  std::mutex mu;           // held by a call from set-up to its last copy: calls on one device serialise
+  double *ptr = nullptr;   // device address of `scratch` on this device
+};
+NaSlot g_na[kNaDevices];
+
+}  // namespace
+
+size_t noalloc_scratch_bytes() { return sizeof(double) * (size_t)kNaDoubles; }
+
+void noalloc_setup(int device) {
+  if (device < 0 || device >= kNaDevices) fail(QM_ERR_RANGE, "device index beyond the scratch table");
+  NaSlot &slot = g_na[device];
+  std::lock_guard<std::mutex> lk(slot.mu);
+  if (slot.ptr) return;
+  void *p = nullptr;
+  // `scratch` names a host-side stand-in; the device copy's address comes from the runtime.
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) {
+    (void)hipGetLastError();
+    fail(QM_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
+  slot.ptr = static_cast<double *>(p);
+}
+
+void reduce_energy_noalloc(State *s, const qm_pauli *terms, int64_t n_terms, double *total,
+                           double *numerators, hipStream_t st) {
+  if (n_terms < 0 || n_terms > kNaTerms) fail(QM_ERR_RANGE, "too many terms for the scratch buffer");
+  const int dev = s->ctx->device;
+  if (dev < 0 || dev >= kNaDevices) fail(QM_ERR_RANGE, "device index beyond the scratch table");
+  NaSlot &slot = g_na[dev];
+  std::lock_guard<std::mutex> lk(slot.mu);
+  if (!slot.ptr) fail(QM_ERR_STATE, "the scratch buffer of this device is not set up");
+  uint64_t per;
+  const int64_t blocks = reduction_blocks((uint64_t)s->length, &per);
+  double *mass_part = slot.ptr;
+  double *term_part = mass_part + kNaBlocks;
+  double *res = term_part + kNaBlocks * kNaTerms;
+  PauliGroup *d_groups = reinterpret_cast<PauliGroup *>(res + 1 + kNaTerms);
+  // Mass: k_mass's masked form with an empty mask sums every probability in
+  // the same order as the total of reduce_mass.
+  if (s->precision == QM_C128)
+    k_mass<C128, 1><<<(int)blocks, kBlock, 0, st>>>((const C128 *)s->data, (uint64_t)s->length, per, 0, 0,
+                                                    mass_part, nullptr);
+  else
+    k_mass<C64, 1><<<(int)blocks, kBlock, 0, st>>>((const C64 *)s->data, (uint64_t)s->length, per, 0, 0,
+                                                   mass_part, nullptr);
+  QM_HIP(hipGetLastError());
+  k_fold<<<1, kBlock, 0, st>>>(mass_part, blocks, 1, 1, res, nullptr, nullptr);
+  QM_HIP(hipGetLastError());
+  // Term groups as reduce_pauli forms them: a stable order by flip mask
+  // (std::sort on (flip, position) pairs, which needs no buffer).
+  std::pair<uint64_t, int64_t> order[kNaTerms];
+  for (int64_t i = 0; i < n_terms; i++) order[i] = {terms[i].flip_mask, i};
+  std::sort(order, order + n_terms);
+  PauliGroup groups[kNaTerms];
+  int64_t ng = 0;
+  for (int64_t k = 0; k < n_terms;) {
+    PauliGroup g{};
+    g.flip = order[k].first;
+    g.first = (int32_t)k;
+    while (k < n_terms && g.count < kTermsPerGroup && order[k].first == g.flip) {
+      g.sign[g.count] = terms[order[k].second].sign_mask;
+      g.y[g.count] = terms[order[k].second].y_count;
+      g.count++;
+      k++;
+    }
+    groups[ng++] = g;
+  }
+  if (ng > 0) {
+    QM_HIP(hipMemcpyAsync(d_groups, groups, (size_t)ng * sizeof(PauliGroup), hipMemcpyHostToDevice, st));
+    dim3 grid((unsigned)blocks, (unsigned)ng);
+    if (s->precision == QM_C128)
+      k_pauli<C128><<<grid, kBlock, 0, st>>>((const C128 *)s->data, (uint64_t)s->length, per, d_groups,
+                                             term_part, blocks);
+    else
+      k_pauli<C64><<<grid, kBlock, 0, st>>>((const C64 *)s->data, (uint64_t)s->length, per, d_groups,
+                                            term_part, blocks);
+    QM_HIP(hipGetLastError());
+    k_fold<<<(unsigned)n_terms, kBlock, 0, st>>>(term_part, blocks, 1, blocks, res + 1, nullptr, nullptr);
+    QM_HIP(hipGetLastError());
+  }
+  double out[1 + kNaTerms];
+  QM_HIP(hipMemcpyAsync(out, res, sizeof(double) * (size_t)(1 + n_terms), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  *total = out[0];
+  for (int64_t k = 0; k < n_terms; k++) numerators[order[k].second] = out[1 + k];
 }
 
 }  // namespace qm

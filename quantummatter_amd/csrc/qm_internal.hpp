@@ -172,6 +172,16 @@ void reduce_pauli(State *s, const qm_pauli *terms, int64_t n_terms, double *nume
                   hipStream_t st);
 void reduce_inner(State *a, State *b, double *re, double *im, hipStream_t st);
 
+// Energy reductions without allocation: the mass and the un-normalised
+// numerators of up to QM_NOALLOC_MAX_TERMS terms, bit-identical to
+// reduce_mass / reduce_pauli, in the device's static scratch array.
+// noalloc_setup (device current) finds that array's device address once per
+// device; calls on one device are serialised by a per-device lock.
+void noalloc_setup(int device);
+size_t noalloc_scratch_bytes();
+void reduce_energy_noalloc(State *s, const qm_pauli *terms, int64_t n_terms, double *total,
+                           double *numerators, hipStream_t st);
+
 // CDF search for a batch of draws with the strict-upper-bound rule; resolves
 // draws within the rounding margin of a boundary with the sequential order.
 // Returns the number of margin fallbacks.
