@@ -37,6 +37,7 @@ HARNESS_HEAD = r'''
 #define __launch_bounds__(a, b)
 #define __shared__
 #define QM_OCC 2
+#define QJ_CONST
 struct Dim { unsigned x; };
 thread_local Dim threadIdx, blockIdx;
 Dim gridDim{1};
@@ -185,9 +186,10 @@ def _variant_case(variant, kind, n, seed, tmp_path):
                                                  (129, "layer", 16, 8), (401, "all", 15, 10),
                                                  (401, "layer", 16, 11), (913, "all", 15, 12),
                                                  (913, "layer", 16, 13), (1425, "all", 15, 14),
-                                                 (1425, "layer", 16, 15), (1937, "all", 15, 16)])
+                                                 (1425, "layer", 16, 15), (1937, "all", 15, 16),
+                                                 (3473, "all", 15, 17), (3985, "layer", 16, 18)])
 def test_generated_kernel_variants_match_oracle(tmp_path, variant, kind, n, seed):
     """Optional code variants: factored forms (bit 4), exchange flags (bit 5),
-    per-tile coefficient loads (bit 7), per-exchange lane addresses (bit 8) and
-    folded LDS offsets (bit 9)."""
+    per-tile coefficient loads (bit 7), per-exchange lane addresses (bit 8),
+    folded LDS offsets (bit 9) and scalar coefficient loads (bit 11)."""
     assert _variant_case(variant, kind, n, seed, tmp_path) <= 1e-12
