@@ -204,6 +204,33 @@ int qm_state_noisy_execute(qm_state *state, const qm_step *steps, int64_t n_step
                            const qm_op *ops, const double *uniforms, int64_t n_uniforms,
                            int64_t *cursor);
 
+/* The caller's random source, for the channel and measurement entry points
+ * that take the reference's System.Random (INoiseChannelBackend.cs:13-14):
+ * called once, on the calling thread, each time the reference calls
+ * Random.NextDouble() -- for amplitude damping only when the jump mass is
+ * positive and the stay mass nonzero (CpuNoiseChannelBackend.cs:117), never at
+ * p == 0 or for a flip at p == 1 -- so the caller's generator advances exactly
+ * as it would under the reference.  Writes one uniform in [0,1) to *value and
+ * returns 0; a nonzero return aborts the call with QM_ERR_POOL. */
+typedef int (*qm_draw_fn)(void *user, double *value);
+
+/* INoiseChannelBackend.Apply (Noise/INoiseChannelBackend.cs:13-14) as
+ * CpuNoiseChannelBackend.Apply (CpuNoiseChannelBackend.cs:10-145) on a resident
+ * state, drawing from the caller's source. */
+int qm_state_noise_draw(qm_state *state, int32_t kind, double probability, int32_t target,
+                        qm_draw_fn draw, void *user);
+/* The drop-in INoiseChannelBackend.Apply on host amplitudes: validates the
+ * length (2^qubits, at least two) and target as CpuNoiseChannelBackend.cs:13-22
+ * does, uploads, applies the channel with the caller's draws, and writes the
+ * conditional state back into `amplitudes` only after success (on failure or
+ * cancellation the buffer is unchanged).  p == 0 returns at once, no draw. */
+int qm_noise_apply(qm_ctx *ctx, int32_t qubits, int32_t kind, double probability, int32_t target,
+                   double *amplitudes, int64_t n_amplitudes, qm_draw_fn draw, void *user,
+                   const volatile int32_t *cancel);
+/* StateVector.EvolveTrajectory (StateVector.cs:74-88) with the caller's draws. */
+int qm_state_noisy_execute_draw(qm_state *state, const qm_step *steps, int64_t n_steps,
+                                const qm_op *ops, qm_draw_fn draw, void *user);
+
 /* Born mass sum |a|^2 (StateVector.cs:109-124), FP64 on device. */
 int qm_state_mass(qm_state *state, double *mass);
 /* StateVector.MeasureAll (StateVector.cs:106-153) with draw u. */
@@ -316,6 +343,14 @@ int qm_state_dynamic_execute(qm_state *state, int32_t classical_bits, const qm_d
                              const int32_t *sel_qubits, const int32_t *sel_cbits, int64_t n_sel,
                              int32_t schedule, const double *uniforms, int64_t n_uniforms,
                              int64_t *cursor, int32_t *classical, const int32_t *cancel);
+/* The same program drawing from the caller's source (one call per Measure and
+ * Reset, the channel rules of qm_state_noise_draw), for a caller whose
+ * System.Random must stay in step with the reference's. */
+int qm_state_dynamic_execute_draw(qm_state *state, int32_t classical_bits, const qm_dyn_step *steps,
+                                  int64_t n_steps, const qm_op *ops, int64_t n_ops,
+                                  const int32_t *sel_qubits, const int32_t *sel_cbits, int64_t n_sel,
+                                  int32_t schedule, qm_draw_fn draw, void *user, int32_t *classical,
+                                  const int32_t *cancel);
 
 /* ---------------------------------------------------------------------------
  * Batched execution: one program over many same-width states, one kernel

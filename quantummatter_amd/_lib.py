@@ -39,6 +39,23 @@ class QmProfile(C.Structure):
                 ("passes", C.c_int32), ("algorithmic_bytes", C.c_double)]
 
 
+# qm_draw_fn: int (*)(void *user, double *value); returns 0 with one uniform in *value.
+QM_DRAW_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double))
+
+
+def draw_callback(random):
+    """A qm_draw_fn over a random source with NextDouble() (System.Random's
+    shape); any exception in the source aborts the library call (QM_ERR_POOL).
+    Keep the returned object alive for the duration of the call."""
+    def fn(_user, out):
+        try:
+            out[0] = float(random.NextDouble())
+            return 0
+        except Exception:  # noqa: BLE001 - reported to the library as a failed draw
+            return 1
+    return QM_DRAW_FN(fn)
+
+
 # name -> argtypes; every function returns int status unless listed in _RESTYPE.
 SIGNATURES = {
     "qm_last_error": [],
@@ -62,6 +79,10 @@ SIGNATURES = {
     "qm_state_execute": [P, P, I64, I32, P, P],
     "qm_state_noise": [P, I32, F64, I32, P, I64, P],
     "qm_state_noisy_execute": [P, P, I64, P, P, I64, P],
+    "qm_state_noise_draw": [P, I32, F64, I32, P, P],
+    "qm_noise_apply": [P, I32, I32, F64, I32, P, I64, P, P, P],
+    "qm_state_noisy_execute_draw": [P, P, I64, P, P, P],
+    "qm_state_dynamic_execute_draw": [P, I32, P, I64, P, I64, P, P, I64, I32, P, P, P, P],
     "qm_state_mass": [P, P],
     "qm_state_measure_all": [P, F64, P],
     "qm_state_measure": [P, P, I32, F64, P],

@@ -174,6 +174,35 @@ class HipStateVectorBackend:
         self.Dispose()
 
 
+class HipNoiseChannelBackend:
+    """INoiseChannelBackend (src/QuantumMatter.Core/Noise/INoiseChannelBackend.cs:13-14)
+    on the device: Apply(channel, target, amplitudes, random) validates like
+    CpuNoiseChannelBackend.Apply (CpuNoiseChannelBackend.cs:13-22), uploads the
+    host amplitudes, samples the branch with the caller's random source
+    (NextDouble(), called exactly when the reference calls it) and writes the
+    conditional state back only after success."""
+
+    def __init__(self, backend: HipStateVectorBackend):
+        if backend is None:
+            raise ArgumentError("backend is null.")
+        self._backend = backend
+
+    def Apply(self, channel: NoiseChannel, target: int, amplitudes: np.ndarray, random,
+              cancellation: Optional[CancellationToken] = None) -> None:
+        if channel is None or random is None:
+            raise ArgumentError("channel and random are required.")
+        if not (isinstance(amplitudes, np.ndarray) and amplitudes.dtype == np.complex128
+                and amplitudes.flags.c_contiguous):
+            raise ArgumentError("amplitudes must be a contiguous complex128 array.")
+        n = amplitudes.size
+        if n < 2 or n & (n - 1):
+            raise ArgumentError("State length must be a power of two, at least two.")
+        cb = _lib.draw_callback(random)
+        call("qm_noise_apply", self._backend._handle(), n.bit_length() - 1, int(channel.Kind),
+             channel.Probability, target, ptr(amplitudes), n, C.cast(cb, C.c_void_p), None,
+             _cancel_ptr(cancellation))
+
+
 def _device_info(i: int) -> HipDeviceInfo:
     info = QmDeviceInfo()
     call("qm_device_get_info", i, C.byref(info))
@@ -273,6 +302,43 @@ class HipStateSession:
         call("qm_state_noise", self._handle(), int(channel.Kind), channel.Probability, target, ptr(u),
              len(u), ptr(cur))
         return int(cur[0])
+
+    def ApplyNoiseRandom(self, channel: NoiseChannel, target: int, random) -> None:
+        """INoiseChannelBackend.Apply (Noise/INoiseChannelBackend.cs:13-14) with the
+        caller's random source (NextDouble()), drawn exactly when
+        CpuNoiseChannelBackend.Apply draws."""
+        if channel is None or random is None:
+            raise ArgumentError("channel and random are required.")
+        cb = _lib.draw_callback(random)
+        call("qm_state_noise_draw", self._handle(), int(channel.Kind), channel.Probability, target,
+             C.cast(cb, C.c_void_p), None)
+
+    def EvolveTrajectoryRandom(self, circuit: NoisyCircuit, random) -> None:
+        """StateVector.EvolveTrajectory (StateVector.cs:74-88) with the caller's random source."""
+        if circuit is None or random is None:
+            raise ArgumentError("circuit and random are required.")
+        if circuit.QubitCount != self.QubitCount:
+            raise ArgumentError("Circuit and state sizes must match.")
+        steps, ops = circuit.packed()
+        cb = _lib.draw_callback(random)
+        call("qm_state_noisy_execute_draw", self._handle(), ptr(steps), len(steps), ptr(ops),
+             C.cast(cb, C.c_void_p), None)
+
+    def EvolveDynamicRandom(self, circuit, random, schedule: int = SCHED_FUSED,
+                            cancellation: Optional[CancellationToken] = None) -> int:
+        """EvolveDynamic / EvolveNoisyDynamic with the caller's random source;
+        returns the classical register."""
+        if circuit is None or random is None:
+            raise ArgumentError("circuit and random are required.")
+        if circuit.QubitCount != self.QubitCount:
+            raise ArgumentError("Circuit and state sizes must match.")
+        steps, ops, sel_q, sel_c = circuit.packed()
+        reg = np.zeros(1, dtype=np.int32)
+        cb = _lib.draw_callback(random)
+        call("qm_state_dynamic_execute_draw", self._handle(), circuit.ClassicalBitCount, ptr(steps), len(steps),
+             ptr(ops), len(ops), ptr(sel_q), ptr(sel_c), len(sel_q), schedule, C.cast(cb, C.c_void_p), None,
+             ptr(reg), _cancel_ptr(cancellation))
+        return int(reg[0])
 
     def EvolveDense(self, circuit: QuantumCircuit, max_block_qubits: int = 4, profile: bool = False):
         """Gates fused into dense blocks of at most max_block_qubits qubits, one

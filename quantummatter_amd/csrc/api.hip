@@ -255,9 +255,46 @@ static State *as_state(qm_state *h) {
   return s;
 }
 
-static void noise_apply(State *s, int kind, double p, int target, const double *u, int64_t n_u,
-                        int64_t *cursor);
 static double take(const double *u, int64_t n, int64_t *cursor);
+
+// The caller's uniform draws (the reference's System.Random): a pool read at a
+// cursor, or a callback asked for one value each time the reference calls
+// Random.NextDouble(), so a caller's own generator stays in step draw for draw.
+struct DrawSource {
+  const double *u = nullptr;
+  int64_t n = 0;
+  int64_t *cursor = nullptr;
+  qm_draw_fn fn = nullptr;
+  void *user = nullptr;
+  double next() {
+    if (!fn) return take(u, n, cursor);
+    double v = 0;
+    if (fn(user, &v) != 0) fail(QM_ERR_POOL, "The draw source failed.");
+    return v;
+  }
+};
+static DrawSource pool_draws(const double *u, int64_t n, int64_t *cursor) {
+  DrawSource d;
+  d.u = u;
+  d.n = n;
+  d.cursor = cursor;
+  return d;
+}
+static DrawSource callback_draws(qm_draw_fn fn, void *user) {
+  if (!fn) fail(QM_ERR_ARG, "draw callback is null");
+  DrawSource d;
+  d.fn = fn;
+  d.user = user;
+  return d;
+}
+
+static void noise_apply(State *s, int kind, double p, int target, DrawSource &draws);
+static void noisy_execute(State *s, const qm_step *steps, int64_t n_steps, const qm_op *ops,
+                          DrawSource &draws);
+static void dynamic_execute(State *s, int32_t classical_bits, const qm_dyn_step *steps, int64_t n_steps,
+                            const qm_op *ops, int64_t n_ops, const int32_t *sel_qubits,
+                            const int32_t *sel_cbits, int64_t n_sel, int32_t schedule, DrawSource &draws,
+                            int32_t *classical, const int32_t *cancel);
 
 }  // namespace qm
 
@@ -1049,8 +1086,57 @@ int qm_state_noise(qm_state *h, int32_t kind, double p, int32_t target, const do
     State *s = as_state(h);
     if (!cursor) fail(QM_ERR_ARG, "cursor is null");
     std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-    noise_apply(s, kind, p, target, u, n_u, cursor);
+    DrawSource d = pool_draws(u, n_u, cursor);
+    noise_apply(s, kind, p, target, d);
     QM_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int qm_state_noise_draw(qm_state *h, int32_t kind, double p, int32_t target, qm_draw_fn draw,
+                        void *user) {
+  return guarded([&] {
+    State *s = as_state(h);
+    DrawSource d = callback_draws(draw, user);
+    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+    noise_apply(s, kind, p, target, d);
+    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+  });
+}
+
+int qm_noise_apply(qm_ctx *h, int32_t qubits, int32_t kind, double p, int32_t target, double *amps,
+                   int64_t n_amps, qm_draw_fn draw, void *user, const volatile int32_t *cancel) {
+  return guarded([&] {
+    Ctx *c = reinterpret_cast<Ctx *>(h);
+    if (!c) fail(QM_ERR_ARG, "context is null");
+    // CpuNoiseChannelBackend.Apply's checks (CpuNoiseChannelBackend.cs:13-22).
+    if (!amps || qubits < 1 || qubits > 34 || n_amps != ((int64_t)1 << qubits))
+      fail(QM_ERR_ARG, "State length must be a power of two, at least two.");
+    if (target < 0 || target >= qubits) fail(QM_ERR_RANGE, "target");
+    if (!std::isfinite(p) || p < 0 || p > 1) fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
+    if (kind < 0 || kind > 3) fail(QM_ERR_ARG, "unknown noise kind");
+    DrawSource d = callback_draws(draw, user);
+    check_cancel(cancel);
+    if (p == 0) return;  // no draw and no change (:24-27)
+    std::lock_guard<std::recursive_mutex> g(c->lock);
+    use_device(c);
+    const int64_t bytes = 16 * n_amps;
+    if ((c->device_budget && bytes > c->device_budget) || bytes > c->device_memory ||
+        (c->staging_budget && bytes > c->staging_budget))
+      fail(QM_ERR_BUDGET, "HIP state exceeds the device or host staging memory budget.");
+    State *s = state_new(c, qubits, QM_C128);
+    try {
+      QM_HIP(hipMemcpyAsync(s->data, amps, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+      noise_apply(s, kind, p, target, d);
+      QM_HIP(hipStreamSynchronize(c->stream));
+      check_cancel(cancel);
+      // The caller's buffer is written only after the branch is applied.
+      QM_HIP(hipMemcpyAsync(amps, s->data, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+      QM_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      state_free(s);
+      throw;
+    }
+    state_free(s);
   });
 }
 
@@ -1058,29 +1144,18 @@ int qm_state_noisy_execute(qm_state *h, const qm_step *steps, int64_t n_steps, c
                            const double *u, int64_t n_u, int64_t *cursor) {
   return guarded([&] {
     State *s = as_state(h);
-    if (!cursor || n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "buffers are null");
-    std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
-    // Consecutive unitary segments run as one fused sequence.
-    std::vector<DevGate> run;
-    auto flush = [&] {
-      if (!run.empty()) run_gates(s, run, QM_SCHED_FUSED, nullptr);
-      run.clear();
-    };
-    for (int64_t k = 0; k < n_steps; k++) {
-      const qm_step &st = steps[k];
-      if (!st.is_channel) {
-        if (st.op_begin < 0 || st.op_count < 0) fail(QM_ERR_ARG, "bad segment");
-        std::vector<DevGate> g2 = prepare_all(ops + st.op_begin, st.op_count, s->qubits);
-        run.insert(run.end(), g2.begin(), g2.end());
-      } else if (st.probability != 0) {
-        flush();
-        noise_apply(s, st.noise_kind, st.probability, st.target, u, n_u, cursor);
-      } else if (!std::isfinite(st.probability) || st.probability < 0 || st.probability > 1) {
-        fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
-      }
-    }
-    flush();
-    QM_HIP(hipStreamSynchronize(s->ctx->stream));
+    if (!cursor) fail(QM_ERR_ARG, "buffers are null");
+    DrawSource d = pool_draws(u, n_u, cursor);
+    noisy_execute(s, steps, n_steps, ops, d);
+  });
+}
+
+int qm_state_noisy_execute_draw(qm_state *h, const qm_step *steps, int64_t n_steps, const qm_op *ops,
+                                qm_draw_fn draw, void *user) {
+  return guarded([&] {
+    State *s = as_state(h);
+    DrawSource d = callback_draws(draw, user);
+    noisy_execute(s, steps, n_steps, ops, d);
   });
 }
 
@@ -1091,7 +1166,64 @@ int qm_state_dynamic_execute(qm_state *h, int32_t classical_bits, const qm_dyn_s
                              int32_t *classical, const int32_t *cancel) {
   return guarded([&] {
     State *s = as_state(h);
-    if (!cursor || !classical || n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "buffers are null");
+    if (!cursor) fail(QM_ERR_ARG, "buffers are null");
+    DrawSource d = pool_draws(u, n_u, cursor);
+    dynamic_execute(s, classical_bits, steps, n_steps, ops, n_ops, sel_qubits, sel_cbits, n_sel, schedule, d,
+                    classical, cancel);
+  });
+}
+
+int qm_state_dynamic_execute_draw(qm_state *h, int32_t classical_bits, const qm_dyn_step *steps,
+                                  int64_t n_steps, const qm_op *ops, int64_t n_ops,
+                                  const int32_t *sel_qubits, const int32_t *sel_cbits, int64_t n_sel,
+                                  int32_t schedule, qm_draw_fn draw, void *user, int32_t *classical,
+                                  const int32_t *cancel) {
+  return guarded([&] {
+    State *s = as_state(h);
+    DrawSource d = callback_draws(draw, user);
+    dynamic_execute(s, classical_bits, steps, n_steps, ops, n_ops, sel_qubits, sel_cbits, n_sel, schedule, d,
+                    classical, cancel);
+  });
+}
+
+}  // extern "C"
+
+namespace qm {
+
+// NoisyCircuit.Execute (NoisyCircuit.cs:48-65) on a resident state.
+static void noisy_execute(State *s, const qm_step *steps, int64_t n_steps, const qm_op *ops,
+                          DrawSource &draws) {
+  if (n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "buffers are null");
+  std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
+  // Consecutive unitary segments run as one fused sequence.
+  std::vector<DevGate> run;
+  auto flush = [&] {
+    if (!run.empty()) run_gates(s, run, QM_SCHED_FUSED, nullptr);
+    run.clear();
+  };
+  for (int64_t k = 0; k < n_steps; k++) {
+    const qm_step &st = steps[k];
+    if (!st.is_channel) {
+      if (st.op_begin < 0 || st.op_count < 0) fail(QM_ERR_ARG, "bad segment");
+      std::vector<DevGate> g2 = prepare_all(ops + st.op_begin, st.op_count, s->qubits);
+      run.insert(run.end(), g2.begin(), g2.end());
+    } else if (st.probability != 0) {
+      flush();
+      noise_apply(s, st.noise_kind, st.probability, st.target, draws);
+    } else if (!std::isfinite(st.probability) || st.probability < 0 || st.probability > 1) {
+      fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
+    }
+  }
+  flush();
+  QM_HIP(hipStreamSynchronize(s->ctx->stream));
+}
+
+// DynamicCircuit.Execute / NoisyDynamicCircuit.Execute on a resident state.
+static void dynamic_execute(State *s, int32_t classical_bits, const qm_dyn_step *steps, int64_t n_steps,
+                            const qm_op *ops, int64_t n_ops, const int32_t *sel_qubits,
+                            const int32_t *sel_cbits, int64_t n_sel, int32_t schedule, DrawSource &draws,
+                            int32_t *classical, const int32_t *cancel) {
+    if (!classical || n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "buffers are null");
     if (classical_bits < 1 || classical_bits > 30) fail(QM_ERR_RANGE, "classicalBits");
     if (schedule < QM_SCHED_PER_GATE || schedule > QM_SCHED_COMPILED) fail(QM_ERR_ARG, "unknown schedule");
     const int n = s->qubits;
@@ -1157,7 +1289,7 @@ int qm_state_dynamic_execute(qm_state *h, int32_t classical_bits, const qm_dyn_s
           run.insert(run.end(), seg[(size_t)k].begin(), seg[(size_t)k].end());
       } else if (st.kind == QM_DYN_MEASURE) {
         flush();
-        const double draw = take(u, n_u, cursor);
+        const double draw = draws.next();
         const int64_t out = measure_subset(s, sel_qubits + st.begin, st.count, draw);
         int32_t written = 0, write_mask = 0;
         for (int b = 0; b < st.count; b++) {
@@ -1167,7 +1299,7 @@ int qm_state_dynamic_execute(qm_state *h, int32_t classical_bits, const qm_dyn_s
         reg = (reg & ~write_mask) | written;
       } else if (st.kind == QM_DYN_RESET) {
         flush();
-        const double draw = take(u, n_u, cursor);
+        const double draw = draws.next();
         const int32_t q = st.target;
         if (measure_subset(s, &q, 1, draw) != 0) {
           qm_op x{};
@@ -1180,19 +1312,14 @@ int qm_state_dynamic_execute(qm_state *h, int32_t classical_bits, const qm_dyn_s
         }
       } else if (st.probability != 0) {
         flush();
-        noise_apply(s, st.noise_kind, st.probability, st.target, u, n_u, cursor);
+        noise_apply(s, st.noise_kind, st.probability, st.target, draws);
       }
     }
     flush();
     QM_HIP(hipStreamSynchronize(s->ctx->stream));
     check_cancel(cancel);
     *classical = reg;
-  });
 }
-
-}  // extern "C"
-
-namespace qm {
 
 static double take(const double *u, int64_t n, int64_t *cursor) {
   if (!u || *cursor >= n) fail(QM_ERR_POOL, "The uniform pool is exhausted.");
@@ -1200,8 +1327,7 @@ static double take(const double *u, int64_t n, int64_t *cursor) {
 }
 
 // CpuNoiseChannelBackend.Apply (CpuNoiseChannelBackend.cs:10-145) on a resident state.
-static void noise_apply(State *s, int kind, double p, int target, const double *u, int64_t n_u,
-                        int64_t *cursor) {
+static void noise_apply(State *s, int kind, double p, int target, DrawSource &draws) {
   if (target < 0 || target >= s->qubits) fail(QM_ERR_RANGE, "target");
   if (!std::isfinite(p) || p < 0 || p > 1)
     fail(QM_ERR_RANGE, "Channel probability must be finite and in [0,1].");
@@ -1220,7 +1346,7 @@ static void noise_apply(State *s, int kind, double p, int target, const double *
     } else if (stay == 0) {
       do_jump = true;
     } else {
-      const double r = take(u, n_u, cursor);
+      const double r = draws.next();
       const double ratio = jump / total;
       if (std::fabs(r - ratio) <= 1e-12 * ratio) {
         // Within the summation-order margin: redo the masses in the
@@ -1238,11 +1364,11 @@ static void noise_apply(State *s, int kind, double p, int target, const double *
   }
   int pauli;
   if (kind == 2) {
-    const double r = take(u, n_u, cursor);
+    const double r = draws.next();
     const double quarter = p / 4;
     pauli = r < quarter ? 1 : r < 2 * quarter ? 2 : r < 3 * quarter ? 3 : 0;
   } else {
-    const bool apply = p == 1 || take(u, n_u, cursor) < p;
+    const bool apply = p == 1 || draws.next() < p;
     pauli = !apply ? 0 : kind == 0 ? 1 : 3;
   }
   if (pauli == 0) return;
