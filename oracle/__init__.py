@@ -54,6 +54,7 @@ def lib():
             "qmo_dynamic_execute": [I32, P, I64, P, P, P, P, P, I64, P, P],
             "qmo_energy_bound": [I32, P, P, I64, P, I64, P, P, I64, P],
             "qmo_parallel_execute": [I32, P, I64, P, I32, I32, I32, I32],
+            "qmo_parallel_execute_ex": [I32, P, I64, P, I32, I32, I32, I32, I32],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -89,12 +90,13 @@ def execute(n: int, ops: np.ndarray, state: np.ndarray = None) -> np.ndarray:
 
 
 def parallel_execute(n: int, ops: np.ndarray, state: np.ndarray = None, workers: int = 8,
-                     simd: bool = True, avx512: bool = True, blocking: bool = True) -> np.ndarray:
+                     simd: bool = True, avx512: bool = True, blocking: bool = True,
+                     min_parallel_qubits: int = 16) -> np.ndarray:
     """The parallel (AVX-512, blocked) CPU baseline on a copy of `state`."""
     s = zero_state(n) if state is None else np.array(state, dtype=np.complex128, copy=True)
     ops = np.ascontiguousarray(ops)
-    _check(lib().qmo_parallel_execute(n, _p(ops), len(ops), _p(s), workers, int(simd), int(avx512),
-                                      int(blocking)), "parallel_execute")
+    _check(lib().qmo_parallel_execute_ex(n, _p(ops), len(ops), _p(s), workers, int(simd), int(avx512),
+                                         int(blocking), int(min_parallel_qubits)), "parallel_execute")
     return s
 
 

@@ -1,4 +1,7 @@
-"""The parallel AVX-512 CPU baseline (oracle/cpu_parallel.cpp) equals the scalar oracle."""
+"""The parallel CPU baseline (oracle/cpu_parallel.cpp) equals the scalar oracle bit for
+bit under every dispatch mode of ParallelCpuStateVectorBackend.cs:246-276: AVX-512
+(simd + avx512), 256-bit AVX with the SIMD Hadamard (simd only), and scalar, with
+and without blocking."""
 import numpy as np
 import pytest
 
@@ -8,6 +11,7 @@ from test_plan_host import random_all_kinds
 
 
 @pytest.mark.parametrize("simd,avx512,blocking", [(True, True, True), (True, True, False),
+                                                  (True, False, True), (True, False, False),
                                                   (False, False, True), (False, False, False)])
 @pytest.mark.parametrize("workers", [1, 4])
 def test_parallel_baseline_matches_scalar(simd, avx512, blocking, workers):
@@ -30,3 +34,21 @@ def test_blocking_on_high_targets():
     exp = oracle.execute(n, ops, psi)
     got = oracle.parallel_execute(n, ops, psi, workers=2)
     assert np.array_equal(got, exp)
+
+
+def test_avx_paths_on_q0_q1_and_unit_halves():
+    """The AVX kernel's q0 lane-permute path, H on q0 and q1 (SIMD Hadamard above
+    q0), diagonals with identity halves (Z, S, T on q1..q3, skipped), odd ranges."""
+    from quantummatter_amd import QuantumCircuit
+    n = 9
+    c = QuantumCircuit(n)
+    for k in range(6):
+        c.H(0).X(0).Rx(0, 0.3 + k).Ry(0, -0.7).Rz(0, 1.1).T(0).S(0).Z(0)
+        c.H(1).Rz(1, 0.2 * k).T(1).S(2).Z(3).Phase(2, 0.4).Y(1).H(4).X(2)
+    ops = c.to_ops()
+    rng = np.random.default_rng(4)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    exp = oracle.execute(n, ops, psi)
+    for avx512 in (False, True):
+        got = oracle.parallel_execute(n, ops, psi, workers=1, simd=True, avx512=avx512, blocking=False)
+        assert np.array_equal(got, exp)
