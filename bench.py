@@ -408,20 +408,29 @@ def batched_extras(be):
 
 def dense_extras(be, n=30, layers=4):
     """The dense-block variant: gates fused into blocks of k <= 5 qubits, one
-    FP64 matrix-core pass per block, on the same generator (complex128)."""
+    matrix-core pass per block (FP64 MFMA in complex128, FP32 MFMA in
+    complex64), on the same generator; the complex64 rows carry their
+    infidelity against the complex128 dense state of the same circuit."""
     from quantummatter_amd import workloads
-    from quantummatter_amd.backend import C128
+    from quantummatter_amd.backend import C64, C128
     c = workloads.random_layer_circuit(n, layers, seed=1)
     out = {}
-    with be.CreateSession(n, C128) as s:
+    peaks = {C128: 78.6e12, C64: 157.3e12}
+    with be.CreateSession(n, C128) as s, be.CreateSession(n, C64) as s64:
         for k in (2, 3, 4, 5):
-            s.Reset()
-            p, flops = s.EvolveDense(c, k, profile=True)
-            t = p.DeviceEvolutionMs * 1e-3
-            out[f"k{k}"] = {"gate_layers_per_s": layers / t, "passes": p.Passes,
-                            "ms_per_pass": p.DeviceEvolutionMs / p.Passes,
-                            "algorithmic_GBps": p.AlgorithmicBytes / t / 1e9,
-                            "mfma_TFLOPs": flops / t / 1e12, "mfma_frac_fp64_peak": flops / t / 78.6e12}
+            for name, st in (("c128", s), ("c64", s64)):
+                prec = C128 if name == "c128" else C64
+                st.Reset()
+                p, flops = st.EvolveDense(c, k, profile=True)
+                t = p.DeviceEvolutionMs * 1e-3
+                row = {"gate_layers_per_s": layers / t, "passes": p.Passes,
+                       "ms_per_pass": p.DeviceEvolutionMs / p.Passes,
+                       "algorithmic_GBps": p.AlgorithmicBytes / t / 1e9,
+                       "mfma_TFLOPs": flops / t / 1e12, "mfma_frac_peak": flops / t / peaks[prec],
+                       "mfma_peak_TFLOPs": peaks[prec] / 1e12}
+                if name == "c64":
+                    row["infidelity_vs_c128"] = 1.0 - abs(st.Inner(s)) ** 2 / (st.Mass() * s.Mass())
+                out[f"k{k}_{name}"] = row
     return out
 
 

@@ -290,7 +290,9 @@ int qm_state_boundary_fallbacks(qm_state *state, int64_t *count);
 
 /* Gates fused into dense blocks of at most max_block_qubits (1..5) qubits,
  * each block one pass of a 2^(k+1)-square real matrix product on the matrix
- * cores (FP64 MFMA), complex128 only.  The reference has no fused form; this
+ * cores: FP64 MFMA on a complex128 state, FP32 MFMA on a complex64 state (the
+ * block's matrix formed in FP64 on the host and rounded to FP32).  The
+ * reference has no fused form; this
  * is the dense-block variant of IStateVectorBackend.Execute
  * (src/QuantumMatter.Core/IStateVectorBackend.cs:13) on a resident state. */
 int qm_state_execute_dense(qm_state *state, const qm_op *ops, int64_t n_ops, int32_t max_block_qubits,

@@ -843,18 +843,28 @@ int qm_state_execute_dense(qm_state *h, const qm_op *ops, int64_t n_ops, int32_t
     if (profile && !c->profiling)
       fail(QM_ERR_STATE, "EnableProfiling must be selected when creating the HIP backend.");
     if (max_block_qubits < 1 || max_block_qubits > 5) fail(QM_ERR_RANGE, "block width must be 1..5");
-    if (s->precision != QM_C128) fail(QM_ERR_UNSUPPORTED, "dense blocks run on complex128 states");
     std::lock_guard<std::recursive_mutex> g(c->lock);
     std::vector<DevGate> gates = prepare_all(ops, n_ops, s->qubits);
     std::vector<DenseBlock> blocks = plan_dense(gates, s->qubits, std::min<int>(max_block_qubits, s->qubits));
-    std::vector<double> host;
-    std::vector<int64_t> at;
+    // Block matrices in the state's precision: FP64 products for complex128,
+    // FP32 (the block multiplied in FP64 on the host, then rounded) for complex64.
+    const bool c64 = s->precision == QM_C64;
+    const size_t elem = c64 ? sizeof(float) : sizeof(double);
+    std::vector<char> host;
+    std::vector<size_t> at;
     for (const DenseBlock &b : blocks) {
-      at.push_back((int64_t)host.size());
-      host.insert(host.end(), b.a.begin(), b.a.end());
+      at.push_back(host.size());
+      const size_t off = host.size();
+      host.resize(off + b.a.size() * elem);
+      if (c64) {
+        float *f = reinterpret_cast<float *>(host.data() + off);
+        for (size_t i = 0; i < b.a.size(); i++) f[i] = (float)b.a[i];
+      } else {
+        std::memcpy(host.data() + off, b.a.data(), b.a.size() * sizeof(double));
+      }
     }
-    const int64_t bytes = (int64_t)(host.size() * sizeof(double));
-    double *d = (double *)ctx_program(c, bytes + 256);
+    const int64_t bytes = (int64_t)host.size();
+    char *d = (char *)ctx_program(c, bytes + 256);
     if (bytes) QM_HIP(hipMemcpyAsync(d, host.data(), (size_t)bytes, hipMemcpyHostToDevice, c->stream));
     if (profile) QM_HIP(hipEventRecord(c->ev_start, c->stream));
     double flops = 0;

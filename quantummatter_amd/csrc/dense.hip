@@ -1,25 +1,31 @@
-// Dense-block passes on the matrix cores (gfx950 FP64 MFMA).
+// Dense-block passes on the matrix cores (gfx950 MFMA: FP64 for complex128
+// states, FP32 for complex64 states).
 //
 // A block of consecutive gates on k <= 5 qubits is one 2^k x 2^k complex
 // matrix U.  Viewing the state as 2^(n-k) columns of 2^k amplitudes (the
 // block qubits varying within a column), the pass computes Y = U X.  In real
 // form, [Re Y; Im Y] = [[Re U, -Im U], [Im U, Re U]] [Re X; Im X]: an
 // M x M by M x (columns) product, M = 2^(k+1), done 16 columns at a time per
-// wavefront with v_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16 per issue).
-// The matrix sits in LDS; each lane loads the four real values of the B
-// operand it owns straight from the state and stores the D rows it holds.
+// wavefront with v_mfma_f64_16x16x4f64 or v_mfma_f32_16x16x4f32 (A 16x4,
+// B 4x16, D 16x16 per issue).  The matrix sits in LDS.  Row r and row r + 2^k
+// of the real form are the real and imaginary parts of the same amplitude,
+// so for k >= 2 each lane loads whole complex amplitudes (one 16- or 8-byte
+// access for two B rows) and, where the output layout puts both parts in the
+// same lane, stores whole complex amplitudes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <vector>
 
 #include "dense.hpp"
+#include "dev_types.hpp"
 #include "qm_internal.hpp"
 
 namespace qm {
 namespace {
 
 typedef double qm_double4 __attribute__((ext_vector_type(4)));
+typedef float qm_float4 __attribute__((ext_vector_type(4)));
 
 struct DenseArgs {
   uint64_t ncols;         // 2^(n - k)
@@ -28,17 +34,51 @@ struct DenseArgs {
   uint8_t run_src[8], run_dst[8], run_len[8];
 };
 
-// Lane layout of v_mfma_f64_16x16x4f64 (established on the device by a probe
-// that matched every output against the product): lane l supplies
-// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[l / 16 + 4 i][l % 16]
-// in register i = 0..3.
-template <int K>
-__global__ __launch_bounds__(256) void k_dense(double *__restrict__ s, const double *__restrict__ A,
+// Lane layouts (lane l, output register i): both instructions take
+// A[l % 16][l / 16] and B[l / 16][l % 16] from lane l.  The FP64 instruction
+// leaves D[l / 16 + 4 i][l % 16] in register i (established on the device by a
+// probe that matched every output against the product); the FP32 one leaves
+// D[4 (l / 16) + i][l % 16] (checked against the host reference by the
+// complex64 dense tests).
+template <typename T> struct DenseMfma;
+template <> struct DenseMfma<C128> {
+  typedef double R;
+  typedef qm_double4 Acc;
+  static __device__ __forceinline__ Acc mfma(R a, R b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static constexpr __host__ __device__ int row(int mt, int grp, int i) { return 16 * mt + grp + 4 * i; }
+};
+template <> struct DenseMfma<C64> {
+  typedef float R;
+  typedef qm_float4 Acc;
+  static __device__ __forceinline__ Acc mfma(R a, R b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static constexpr __host__ __device__ int row(int mt, int grp, int i) { return 16 * mt + 4 * grp + i; }
+};
+
+// Where the imaginary part of output row `row` (< D) sits: the register and
+// tile that hold row + D in the same lane, or -1 when it is in another lane.
+template <typename T, int K>
+__device__ __forceinline__ constexpr int partner_reg(int mt, int i) {
+  constexpr int D = 1 << K;
+  if (D >= 16) return i;  // tile mt + D / 16, same register
+  if (DenseMfma<T>::row(0, 0, 1) == 4) return i + D / 4;  // FP64 layout: rows step by 4 per register
+  return -1;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_dense(T *__restrict__ s, const typename DenseMfma<T>::R *__restrict__ A,
                                                const DenseArgs args) {
+  using Mf = DenseMfma<T>;
+  using R = typename Mf::R;
+  using Acc = typename Mf::Acc;
   constexpr int D = 1 << K, M = 2 * D, MT = (M + 15) / 16, KS = M / 4;
-  __shared__ double a_lds[M * M];
+  __shared__ R a_lds[M * M];
   for (int i = threadIdx.x; i < M * M; i += blockDim.x) a_lds[i] = A[i];
   __syncthreads();
+  R *const sr = reinterpret_cast<R *>(s);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = lane & 15, grp = lane >> 4;
   const uint64_t tiles = (args.ncols + 15) / 16;
@@ -49,51 +89,86 @@ __global__ __launch_bounds__(256) void k_dense(double *__restrict__ s, const dou
     uint64_t base = 0;
     for (int r = 0; r < args.nruns; r++)
       base |= ((col >> args.run_src[r]) & ((1ull << args.run_len[r]) - 1)) << args.run_dst[r];
-    double b[KS];
+    R b[KS];
+    if constexpr (K >= 2) {
+      // B rows 4 kk + grp (kk < KS / 2) are the real parts of local index
+      // 4 kk + grp, rows 4 kk + grp + D their imaginary parts.
 #pragma unroll
-    for (int kk = 0; kk < KS; kk++) {
-      const int row = 4 * kk + grp;  // < M
-      const int local = row & (D - 1), part = row >> K;
-      b[kk] = valid ? s[2 * (base | args.loc_off[local]) + part] : 0.0;
+      for (int kk = 0; kk < KS / 2; kk++) {
+        T v{0, 0};
+        if (valid) v = s[base | args.loc_off[4 * kk + grp]];
+        b[kk] = v.re;
+        b[kk + KS / 2] = v.im;
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < KS; kk++) {
+        const int row = 4 * kk + grp;
+        const int local = row & (D - 1), part = row >> K;
+        b[kk] = valid ? sr[2 * (base | args.loc_off[local]) + part] : (R)0;
+      }
     }
-    qm_double4 acc[MT];
+    Acc acc[MT];
 #pragma unroll
-    for (int mt = 0; mt < MT; mt++) acc[mt] = qm_double4{0, 0, 0, 0};
+    for (int mt = 0; mt < MT; mt++) acc[mt] = Acc{0, 0, 0, 0};
 #pragma unroll
     for (int kk = 0; kk < KS; kk++) {
 #pragma unroll
       for (int mt = 0; mt < MT; mt++) {
         const int arow = 16 * mt + cl;
-        const double a = arow < M ? a_lds[arow * M + 4 * kk + grp] : 0.0;
-        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[kk], acc[mt], 0, 0, 0);
+        const R a = arow < M ? a_lds[arow * M + 4 * kk + grp] : (R)0;
+        acc[mt] = Mf::mfma(a, b[kk], acc[mt]);
       }
     }
+    if (!valid) continue;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int row = 16 * mt + grp + 4 * i;
-        if (row < M && valid) {
+        const int row = Mf::row(mt, grp, i);
+        if (row >= M) continue;
+        constexpr bool paired = K >= 2 && partner_reg<T, K>(0, 0) >= 0;
+        if constexpr (paired) {
+          if (row >= D) continue;  // written with its real part
+          const int mt2 = D >= 16 ? mt + D / 16 : mt;
+          const int i2 = partner_reg<T, K>(mt, i);
+          T v;
+          v.re = acc[mt][i];
+          v.im = acc[mt2][i2];
+          s[base | args.loc_off[row]] = v;
+        } else {
           const int local = row & (D - 1), part = row >> K;
-          s[2 * (base | args.loc_off[local]) + part] = acc[mt][i];
+          sr[2 * (base | args.loc_off[local]) + part] = acc[mt][i];
         }
       }
     }
   }
 }
 
-template <int K>
-void launch_dense_t(State *st, const DenseArgs &a, const double *d_matrix, hipStream_t stream) {
+template <typename T, int K>
+void launch_dense_t(State *st, const DenseArgs &a, const void *d_matrix, hipStream_t stream) {
   const uint64_t tiles = (a.ncols + 15) / 16;
   const uint64_t blocks = std::min<uint64_t>((tiles + 3) / 4, (uint64_t)st->ctx->compute_units * 8);
-  k_dense<K><<<(unsigned)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>((double *)st->data, d_matrix, a);
+  k_dense<T, K><<<(unsigned)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(
+      (T *)st->data, (const typename DenseMfma<T>::R *)d_matrix, a);
   QM_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_dense_k(State *st, int k, const DenseArgs &a, const void *d_matrix, hipStream_t stream) {
+  switch (k) {
+    case 1: launch_dense_t<T, 1>(st, a, d_matrix, stream); break;
+    case 2: launch_dense_t<T, 2>(st, a, d_matrix, stream); break;
+    case 3: launch_dense_t<T, 3>(st, a, d_matrix, stream); break;
+    case 4: launch_dense_t<T, 4>(st, a, d_matrix, stream); break;
+    case 5: launch_dense_t<T, 5>(st, a, d_matrix, stream); break;
+    default: fail(QM_ERR_UNSUPPORTED, "dense block width must be 1..5");
+  }
 }
 
 }  // namespace
 
-void launch_dense(State *st, const DenseBlock &b, const double *d_matrix, hipStream_t stream) {
-  if (st->precision != QM_C128) fail(QM_ERR_UNSUPPORTED, "dense blocks run on complex128 states");
+void launch_dense(State *st, const DenseBlock &b, const void *d_matrix, hipStream_t stream) {
   DenseArgs a{};
   const int n = st->qubits;
   a.ncols = 1ull << (n - b.k);
@@ -123,14 +198,10 @@ void launch_dense(State *st, const DenseBlock &b, const double *d_matrix, hipStr
     l++;
   }
   a.nruns = l;
-  switch (b.k) {
-    case 1: launch_dense_t<1>(st, a, d_matrix, stream); break;
-    case 2: launch_dense_t<2>(st, a, d_matrix, stream); break;
-    case 3: launch_dense_t<3>(st, a, d_matrix, stream); break;
-    case 4: launch_dense_t<4>(st, a, d_matrix, stream); break;
-    case 5: launch_dense_t<5>(st, a, d_matrix, stream); break;
-    default: fail(QM_ERR_UNSUPPORTED, "dense block width must be 1..5");
-  }
+  if (st->precision == QM_C128)
+    launch_dense_k<C128>(st, b.k, a, d_matrix, stream);
+  else
+    launch_dense_k<C64>(st, b.k, a, d_matrix, stream);
 }
 
 }  // namespace qm

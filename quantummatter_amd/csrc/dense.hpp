@@ -9,8 +9,9 @@ namespace qm {
 
 struct State;
 
-// Applies one dense block to a complex128 state; d_matrix holds the block's
-// real (2^(k+1))^2 matrix in device memory.
-void launch_dense(State *st, const DenseBlock &b, const double *d_matrix, hipStream_t stream);
+// Applies one dense block to a state; d_matrix holds the block's real
+// (2^(k+1))^2 matrix in device memory, as doubles for a complex128 state and
+// as floats for a complex64 state.
+void launch_dense(State *st, const DenseBlock &b, const void *d_matrix, hipStream_t stream);
 
 }  // namespace qm

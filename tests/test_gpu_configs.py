@@ -102,6 +102,17 @@ def test_c2_dense_blocks_30q(k):
         assert abs(b.Mass() - 1) <= 1e-12
 
 
+def test_c2_dense_blocks_c64_30q(c2_per_gate):
+    """Dense k = 5 blocks on a 30-qubit complex64 state (FP32 matrix cores),
+    2 layers, against the complex128 per-gate state of the same 2 layers."""
+    n = 30
+    c = workloads.random_layer_circuit(n, 2, seed=31)
+    with shared_backend().CreateSession(n, C128) as a, shared_backend().CreateSession(n, C64) as b:
+        a.EvolveOps(c.to_ops(), SCHED_PER_GATE)
+        b.EvolveDense(c, 5)
+        assert 1 - abs(b.Inner(a)) ** 2 / b.Mass() <= 1e-5
+
+
 # ---- variational energy and gradient ---------------------------------------------
 
 
