@@ -48,6 +48,10 @@ extern "C" {
 #define QM_SCHED_FUSED 1    /* register/LDS tile passes: many gates per HBM pass (default) */
 #define QM_SCHED_COMPILED 2 /* the same passes compiled to straight-line kernels (hipRTC), cached
                                by circuit structure: the first run of a structure compiles */
+#define QM_SCHED_AUTO 3     /* compiled kernels when the structure's are all cached; otherwise the
+                               interpreting kernel now, and the structure's kernels compiled in the
+                               background for its next run (QM_AUTO_COMPILE=0: no background
+                               compilation).  qm_execute uses this schedule. */
 
 /* GateOperation (src/QuantumMatter.Core/QuantumCircuit.cs:49-61).  kind uses
  * the GateKind numbering (:4-40); control/second_control are -1 when absent;
@@ -306,6 +310,10 @@ int qm_plan_dense_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, in
 /* Pass kernels compiled for QM_SCHED_COMPILED so far in this process, cache
  * hits, and the wall time spent compiling (no counterpart in the reference). */
 int qm_jit_stats(int64_t *compiled, int64_t *cache_hits, double *compile_seconds);
+/* Blocks until the background compilations that QM_SCHED_AUTO started have
+ * finished (no counterpart in the reference: a caller that wants the next run
+ * of a structure to be compiled, e.g. before timing it). */
+int qm_jit_drain(void);
 /* Generates and compiles (without loading) the pass kernels of a plan for the
  * given tile shape; *passes = kernels compiled.  Needs no device. */
 int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n_ops,
@@ -461,6 +469,9 @@ typedef struct qm_server_options {
   int32_t batch_max_qubits;    /* widths batched (0: 14) */
   int32_t max_batch;           /* states per batched launch (0: 1024) */
   int64_t device_budget_bytes; /* admission budget (0: device memory less 4 GiB) */
+  int32_t schedule;            /* full-chip lanes: 0 or QM_SCHED_FUSED, or QM_SCHED_AUTO for
+                                  streams that repeat circuit structures */
+  int32_t pad;
 } qm_server_options;
 
 typedef struct qm_request {

@@ -106,6 +106,7 @@ SIGNATURES = {
     "qm_vqe_program_energies": [P, P, I64, P, P],
     "qm_vqe_program_destroy": [P],
     "qm_jit_stats": [P, P, P],
+    "qm_jit_drain": [],
     "qm_state_execute_dense": [P, P, I64, I32, P],
     "qm_dense_flops": [P, P],
     "qm_plan_dense_apply_host": [I32, P, I64, I32, P, P],

@@ -28,6 +28,19 @@ C64 = 1
 SCHED_PER_GATE = 0
 SCHED_FUSED = 1
 SCHED_COMPILED = 2  # fused passes compiled to straight-line kernels, cached by structure
+SCHED_AUTO = 3  # compiled when the structure's kernels are cached, else interpreted + background compile
+
+
+def jit_drain() -> None:
+    """Waits for the background compilations SCHED_AUTO started (qm_jit_drain)."""
+    call("qm_jit_drain")
+
+
+def jit_stats():
+    """(kernels compiled, cache hits, compile seconds) so far in this process."""
+    a, b, c = C.c_int64(), C.c_int64(), C.c_double()
+    call("qm_jit_stats", C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
 
 
 @dataclass(frozen=True)

@@ -129,37 +129,58 @@ void apply_tuning_knobs(FusedShape &shape) {
 }
 
 // Runs prepared gates on a resident state with the chosen schedule.
+// Whether QM_SCHED_AUTO compiles a structure's kernels in the background on
+// its first (interpreted) execution; tuning knob QM_AUTO_COMPILE=0 turns it off.
+static bool auto_compile() {
+  static const bool on = [] {
+    const char *e = std::getenv("QM_AUTO_COMPILE");
+    return !e || std::atoi(e) != 0;
+  }();
+  return on;
+}
+
 ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in,
                     const volatile int32_t *cancel, const FusedShape *force) {
   Ctx *c = s->ctx;
   ExecStats st;
   const double state_bytes = (double)s->bytes;
-  FusedShape shape = c->shape[s->precision];
-  int schedule = schedule_in;
-  const int tuned_slot = schedule == QM_SCHED_FUSED ? 0 : schedule == QM_SCHED_COMPILED ? 1 : -1;
-  if (force) {
-    shape = *force;
-  } else if (tuned_slot >= 0 && c->tuned[tuned_slot][s->precision][s->qubits].K) {
-    shape = c->tuned[tuned_slot][s->precision][s->qubits];
-  } else if (schedule == QM_SCHED_COMPILED && s->qubits < shape.K && s->qubits >= 12) {
-    const FusedShape wide = shape;
-    // Narrower states: the widest tile of 256 threads that fits (8192 or 4096
-    // amplitudes).
-    const int k = std::min(s->qubits, 13);
-    shape = FusedShape{k, 8, k - 8, wide.contiguous};
-    shape.remap = wide.remap;
-    shape.search = wide.search;
-    shape.max_phases = wide.max_phases;
-  }
-  if (!force && schedule == QM_SCHED_FUSED && !(tuned_slot >= 0 && c->tuned[0][s->precision][s->qubits].K)) {
-    // The interpreting kernel keeps the lowest qubits in place (its passes are
-    // bound by the kernel's own work, not by the access pattern); its tiles are
-    // searched like the compiled schedule's.
-    shape.contiguous = s->precision == QM_C128 ? 3 : 4;
-    shape.remap = 0;
-    shape.search = 1;
-  }
-  if (!force) apply_tuning_knobs(shape);
+  // Tile shape of a schedule: forced, tuned for this width, or the context's
+  // default adjusted for the schedule.
+  auto shape_for = [&](int sched) {
+    FusedShape shape = c->shape[s->precision];
+    const int tuned_slot = sched == QM_SCHED_FUSED ? 0 : sched == QM_SCHED_COMPILED ? 1 : -1;
+    if (force) {
+      shape = *force;
+    } else if (tuned_slot >= 0 && c->tuned[tuned_slot][s->precision][s->qubits].K) {
+      shape = c->tuned[tuned_slot][s->precision][s->qubits];
+    } else if (sched == QM_SCHED_COMPILED && s->qubits < shape.K && s->qubits >= 12) {
+      const FusedShape wide = shape;
+      // Narrower states: the widest tile of 256 threads that fits (8192 or 4096
+      // amplitudes).
+      const int k = std::min(s->qubits, 13);
+      shape = FusedShape{k, 8, k - 8, wide.contiguous};
+      shape.remap = wide.remap;
+      shape.search = wide.search;
+      shape.max_phases = wide.max_phases;
+    }
+    if (!force && sched == QM_SCHED_FUSED && !(tuned_slot >= 0 && c->tuned[0][s->precision][s->qubits].K)) {
+      // The interpreting kernel keeps the lowest qubits in place (its passes are
+      // bound by the kernel's own work, not by the access pattern); its tiles are
+      // searched like the compiled schedule's.
+      shape.contiguous = s->precision == QM_C128 ? 3 : 4;
+      shape.remap = 0;
+      shape.search = 1;
+    }
+    if (!force) apply_tuning_knobs(shape);
+    return shape;
+  };
+  // QM_SCHED_AUTO: the compiled kernels when this structure's are all in the
+  // cache, else the interpreting kernel now and the compilation in the
+  // background, so a structure run again (a parameter sweep, a VQE loop, a
+  // served circuit family) runs compiled from then on.
+  const bool automatic = schedule_in == QM_SCHED_AUTO;
+  int schedule = automatic ? QM_SCHED_COMPILED : schedule_in;
+  FusedShape shape = shape_for(schedule);
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
     FusedPlan plan = plan_fused(gates, s->qubits, shape);
     if (shape.alt_contiguous > 0 && shape.alt_contiguous < shape.K && shape.alt_contiguous != shape.contiguous) {
@@ -189,8 +210,18 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
           }
         }
     }
+    JitProgram jp;
+    bool ready = false;
     if (jit_supported(plan)) {
-      JitProgram jp = jit_prepare(plan, s->precision, c->device);
+      if (!automatic) {
+        jp = jit_prepare(plan, s->precision, c->device);
+        ready = true;
+      } else {
+        ready = jit_lookup(plan, s->precision, c->device, &jp);
+        if (!ready && auto_compile()) jit_prepare_async(plan, s->precision, c->device);
+      }
+    }
+    if (ready) {
       const int64_t bytes = (int64_t)(jp.coef.size() * sizeof(double));
       double *d_coef = (double *)ctx_program(c, bytes + 256);
       if (bytes)
@@ -207,6 +238,10 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
       return st;
     }
     schedule = QM_SCHED_FUSED;
+    shape = shape_for(schedule);
+  } else if (automatic) {
+    schedule = QM_SCHED_FUSED;
+    shape = shape_for(schedule);
   }
   if (schedule == QM_SCHED_FUSED && !fused_shape_supported(s->precision, shape.L, shape.R)) {
     const int search = shape.search;
@@ -469,7 +504,7 @@ int qm_execute(qm_ctx *h, int32_t qubits, const qm_op *ops, int64_t n_ops, doubl
     try {
       QM_HIP(hipMemcpyAsync(s->data, amps, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
       if (profile) QM_HIP(hipEventRecord(c->ev_start, c->stream));
-      ExecStats es = run_gates(s, gates, QM_SCHED_FUSED, cancel);
+      ExecStats es = run_gates(s, gates, QM_SCHED_AUTO, cancel);
       if (profile) QM_HIP(hipEventRecord(c->ev_stop, c->stream));
       QM_HIP(hipStreamSynchronize(c->stream));
       check_cancel(cancel);
@@ -608,7 +643,8 @@ int qm_state_execute(qm_state *h, const qm_op *ops, int64_t n_ops, int32_t sched
     Ctx *c = s->ctx;
     if (profile && !c->profiling)
       fail(QM_ERR_STATE, "EnableProfiling must be selected when creating the HIP backend.");
-    if (schedule != QM_SCHED_PER_GATE && schedule != QM_SCHED_FUSED && schedule != QM_SCHED_COMPILED)
+    if (schedule != QM_SCHED_PER_GATE && schedule != QM_SCHED_FUSED && schedule != QM_SCHED_COMPILED &&
+        schedule != QM_SCHED_AUTO)
       fail(QM_ERR_ARG, "unknown schedule");
     check_cancel(cancel);
     std::lock_guard<std::recursive_mutex> g(c->lock);
@@ -933,6 +969,10 @@ int qm_jit_stats(int64_t *compiled, int64_t *cache_hits, double *compile_seconds
   return guarded([&] { jit_stats(compiled, cache_hits, compile_seconds); });
 }
 
+int qm_jit_drain(void) {
+  return guarded([&] { jit_drain(); });
+}
+
 int qm_state_boundary_fallbacks(qm_state *h, int64_t *count) {
   return guarded([&] {
     State *s = as_state(h);
@@ -1235,7 +1275,7 @@ static void dynamic_execute(State *s, int32_t classical_bits, const qm_dyn_step 
                             int32_t *classical, const int32_t *cancel) {
     if (!classical || n_steps < 0 || (n_steps > 0 && !steps)) fail(QM_ERR_ARG, "buffers are null");
     if (classical_bits < 1 || classical_bits > 30) fail(QM_ERR_RANGE, "classicalBits");
-    if (schedule < QM_SCHED_PER_GATE || schedule > QM_SCHED_COMPILED) fail(QM_ERR_ARG, "unknown schedule");
+    if (schedule < QM_SCHED_PER_GATE || schedule > QM_SCHED_AUTO) fail(QM_ERR_ARG, "unknown schedule");
     const int n = s->qubits;
     const int32_t reg_mask = (int32_t)((1u << classical_bits) - 1);
     // Every instruction is validated before any device work, as the builders

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -904,9 +905,11 @@ struct JitCache {
   std::atomic<int64_t> compile_us{0};
 };
 
+// Held for the life of the process (never destroyed): a background compile
+// may still be using it while static objects are torn down at exit.
 JitCache &cache() {
-  static JitCache c;
-  return c;
+  static JitCache *c = new JitCache;
+  return *c;
 }
 
 // Directory of libqmhip.so, where the build puts qm_jitc beside it.
@@ -1015,10 +1018,10 @@ bool jit_supported(const FusedPlan &plan) {
   return true;
 }
 
-JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
-  check_device_variant(variant_of(plan.shape, precision));
-  JitProgram jp;
-  std::vector<std::string> keys(plan.passes.size());
+// Sources (cache keys) and coefficients of a plan's pass kernels.
+static void plan_keys(const FusedPlan &plan, int precision, int device, JitProgram &jp,
+                      std::vector<std::string> &keys) {
+  keys.resize(plan.passes.size());
   for (size_t i = 0; i < plan.passes.size(); i++) {
     // Coefficient indices in the source are relative to the pass (the source,
     // and so the cache key, does not depend on where the pass sits).
@@ -1028,6 +1031,91 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
     jp.coef.insert(jp.coef.end(), local.begin(), local.end());
     keys[i] = std::to_string(device) + "\n" + kJitPrelude + src;
   }
+}
+
+bool jit_lookup(const FusedPlan &plan, int precision, int device, JitProgram *out) {
+  check_device_variant(variant_of(plan.shape, precision));
+  JitProgram jp;
+  std::vector<std::string> keys;
+  plan_keys(plan, precision, device, jp, keys);
+  JitCache &c = cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (const std::string &k : keys)
+    if (!c.map.count(k)) return false;
+  for (const std::string &k : keys) {
+    jp.fns.push_back((void *)c.map[k].fn);
+    jp.resident.push_back(c.map[k].resident);
+  }
+  c.hits += (int64_t)keys.size();
+  *out = std::move(jp);
+  return true;
+}
+
+namespace {
+// Background compilation of plans whose kernels a later execution will find
+// in the cache: one worker at a time, plans already queued not queued again.
+struct AsyncJit {
+  std::mutex mu;
+  std::condition_variable idle;
+  int running = 0;
+  std::unordered_map<std::string, int> queued;  // by the plan's first pass key + pass count
+};
+AsyncJit &async_jit() {
+  static AsyncJit *a = new AsyncJit;
+  return *a;
+}
+std::mutex &async_serial() {
+  static std::mutex *m = new std::mutex;
+  return *m;
+}
+}  // namespace
+
+void jit_prepare_async(const FusedPlan &plan, int precision, int device) {
+  if (plan.passes.empty()) return;
+  std::string tag;
+  {
+    JitProgram jp;
+    std::vector<std::string> keys;
+    plan_keys(plan, precision, device, jp, keys);
+    for (const std::string &k : keys) tag += std::to_string(std::hash<std::string>{}(k)) + ",";
+  }
+  AsyncJit &a = async_jit();
+  {
+    std::lock_guard<std::mutex> lk(a.mu);
+    if (a.queued.count(tag) || a.running >= 4) return;  // already queued, or the queue is full
+    a.queued[tag] = 1;
+    a.running++;
+  }
+  std::thread([plan, precision, device, tag] {
+    {
+      std::lock_guard<std::mutex> serial(async_serial());
+      try {
+        (void)hipSetDevice(device);
+        (void)jit_prepare(plan, precision, device);
+      } catch (...) {
+        // A failed background compile only means the next execution compiles
+        // (and reports the failure) or runs interpreted.
+      }
+    }
+    AsyncJit &a = async_jit();
+    std::lock_guard<std::mutex> lk(a.mu);
+    a.queued.erase(tag);
+    a.running--;
+    a.idle.notify_all();
+  }).detach();
+}
+
+void jit_drain() {
+  AsyncJit &a = async_jit();
+  std::unique_lock<std::mutex> lk(a.mu);
+  a.idle.wait(lk, [&] { return a.running == 0; });
+}
+
+JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
+  check_device_variant(variant_of(plan.shape, precision));
+  JitProgram jp;
+  std::vector<std::string> keys;
+  plan_keys(plan, precision, device, jp, keys);
   JitCache &c = cache();
   std::vector<size_t> missing;
   {

@@ -22,6 +22,13 @@ struct JitProgram {
 
 bool jit_supported(const FusedPlan &plan);  // no run-time resolved ops
 JitProgram jit_prepare(const FusedPlan &plan, int precision, int device);
+// The plan's kernels if every one is already compiled (no compilation).
+bool jit_lookup(const FusedPlan &plan, int precision, int device, JitProgram *out);
+// Compiles the plan's missing kernels on a background thread (at most a few
+// plans at once; a plan already queued is not queued again).
+void jit_prepare_async(const FusedPlan &plan, int precision, int device);
+// Waits until no background compilation is running.
+void jit_drain();
 void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,
                 const double *d_coef, hipStream_t st);
 void jit_stats(int64_t *compiled, int64_t *hits, double *seconds);

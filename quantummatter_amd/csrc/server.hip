@@ -56,6 +56,7 @@ struct Server {
   int device = 0;
   int64_t budget = 0;
   int batch_max_qubits = 16;
+  int schedule = QM_SCHED_FUSED;  // full-chip lanes
   int max_batch = 1024;
   std::vector<Ctx *> lanes;  // [0] batch lane, [1..] full-chip lanes
   std::mutex mu;
@@ -140,11 +141,11 @@ void download(Ctx *c, void *dst, const void *src, size_t bytes) {
 }
 
 // One request on a full-chip lane: resident state, fused passes, result.
-void run_single(Ctx *c, Req *r) {
+void run_single(Ctx *c, Req *r, int schedule) {
   State *s = state_new(c, r->qubits, r->precision);
   try {
     launch_set_basis(s, 0, c->stream);
-    run_gates(s, r->gates, QM_SCHED_FUSED, nullptr);
+    run_gates(s, r->gates, schedule, nullptr);
     if (r->kind == QM_REQ_AMPLITUDES) {
       download(c, r->out, s->data, (size_t)s->bytes);
     } else if (r->kind == QM_REQ_SHOTS) {
@@ -293,7 +294,7 @@ void lane_full(Server *sv, int lane) {
     int code = QM_OK;
     std::string msg;
     try {
-      run_single(c, r);
+      run_single(c, r, sv->schedule);
     } catch (const Status &s) {
       code = s.code;
       msg = s.msg;
@@ -363,7 +364,7 @@ void lane_batch(Server *sv) {
     }
     for (Req *r : single) {
       try {
-        run_single(c, r);
+        run_single(c, r, sv->schedule);
       } catch (const Status &s) {
         r->status = s.code;
         r->msg = s.msg;
@@ -433,6 +434,9 @@ int qm_server_create(const qm_server_options *opt, qm_server **out) {
     if (o.max_batch < 0) fail(QM_ERR_RANGE, "max_batch must be nonnegative");
     auto sv = std::make_unique<Server>();
     sv->device = o.device_index;
+    if (o.schedule != 0 && o.schedule != QM_SCHED_FUSED && o.schedule != QM_SCHED_AUTO)
+      fail(QM_ERR_ARG, "schedule must be 0, QM_SCHED_FUSED or QM_SCHED_AUTO");
+    sv->schedule = o.schedule == QM_SCHED_AUTO ? QM_SCHED_AUTO : QM_SCHED_FUSED;
     sv->batch_max_qubits = o.batch_max_qubits > 0 ? o.batch_max_qubits : 14;
     sv->max_batch = o.max_batch > 0 ? o.max_batch : 1024;
     const int full = o.full_lanes > 0 ? o.full_lanes : 3;

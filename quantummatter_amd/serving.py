@@ -25,7 +25,7 @@ AMPLITUDES, SHOTS, EXPECTATION = 0, 1, 2
 class QmServerOptions(C.Structure):
     _fields_ = [("device_index", C.c_int32), ("full_lanes", C.c_int32),
                 ("batch_max_qubits", C.c_int32), ("max_batch", C.c_int32),
-                ("device_budget_bytes", C.c_int64)]
+                ("device_budget_bytes", C.c_int64), ("schedule", C.c_int32), ("pad", C.c_int32)]
 
 
 class QmRequest(C.Structure):
@@ -66,8 +66,9 @@ class RequestResult:
 
 class HipRequestServer:
     def __init__(self, device_index: int = 0, full_lanes: int = 3, batch_max_qubits: int = 14,
-                 max_batch: int = 1024, device_budget_bytes: int = 0):
-        o = QmServerOptions(device_index, full_lanes, batch_max_qubits, max_batch, device_budget_bytes)
+                 max_batch: int = 1024, device_budget_bytes: int = 0, schedule: int = 0):
+        o = QmServerOptions(device_index, full_lanes, batch_max_qubits, max_batch, device_budget_bytes,
+                            schedule, 0)
         h = C.c_void_p()
         call("qm_server_create", C.byref(o), C.byref(h))
         self._h = h

@@ -52,3 +52,5 @@ def test_packed_record_layouts():
     from quantummatter_amd.circuit import OP_DTYPE, PAULI_DTYPE, STEP_DTYPE
     assert OP_DTYPE.itemsize == 88 and PAULI_DTYPE.itemsize == 24 and STEP_DTYPE.itemsize == 40
     assert C.sizeof(_lib.QmProfile) == 32
+    from quantummatter_amd.serving import QmServerOptions
+    assert C.sizeof(QmServerOptions) == 32
