@@ -89,6 +89,82 @@ void *ctx_program(Ctx *c, int64_t bytes) {
   return c->d_prog;
 }
 
+// Host <-> device copies of large buffers through two pinned staging chunks of
+// the context: the DMA of one chunk overlaps the host-side copy of the other,
+// which runs the PCIe link faster than copies from pageable caller memory.
+// Synchronous on the context's stream; small buffers are copied directly.
+namespace {
+constexpr size_t kStageChunk = (size_t)8 << 20;   // two of them pinned per context
+constexpr size_t kDirectCopy = (size_t)1 << 20;   // smaller buffers are copied directly
+}
+
+void copy_to_device(Ctx *c, void *dst, const void *src, size_t bytes) {
+  if (bytes <= kDirectCopy) {
+    QM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    QM_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  double *pin = ctx_pinned(c, (int64_t)(2 * kStageChunk / sizeof(double)));
+  char *stage[2] = {reinterpret_cast<char *>(pin), reinterpret_cast<char *>(pin) + kStageChunk};
+  hipEvent_t done[2];
+  QM_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+  QM_HIP(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+  const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
+  try {
+    for (size_t k = 0; k < chunks; k++) {
+      const size_t off = k * kStageChunk, n = std::min(kStageChunk, bytes - off);
+      if (k >= 2) QM_HIP(hipEventSynchronize(done[k & 1]));  // the chunk's buffer is free again
+      std::memcpy(stage[k & 1], static_cast<const char *>(src) + off, n);
+      QM_HIP(hipMemcpyAsync(static_cast<char *>(dst) + off, stage[k & 1], n, hipMemcpyHostToDevice, c->stream));
+      QM_HIP(hipEventRecord(done[k & 1], c->stream));
+    }
+    QM_HIP(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipEventDestroy(done[0]);
+    (void)hipEventDestroy(done[1]);
+    throw;
+  }
+  QM_HIP(hipEventDestroy(done[0]));
+  QM_HIP(hipEventDestroy(done[1]));
+}
+
+void copy_to_host(Ctx *c, void *dst, const void *src, size_t bytes) {
+  if (bytes <= kDirectCopy) {
+    QM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    QM_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  double *pin = ctx_pinned(c, (int64_t)(2 * kStageChunk / sizeof(double)));
+  char *stage[2] = {reinterpret_cast<char *>(pin), reinterpret_cast<char *>(pin) + kStageChunk};
+  hipEvent_t done[2];
+  QM_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+  QM_HIP(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+  const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t k) {
+    const size_t off = k * kStageChunk, n = std::min(kStageChunk, bytes - off);
+    QM_HIP(hipMemcpyAsync(stage[k & 1], static_cast<const char *>(src) + off, n, hipMemcpyDeviceToHost,
+                          c->stream));
+    QM_HIP(hipEventRecord(done[k & 1], c->stream));
+  };
+  try {
+    issue(0);
+    for (size_t k = 0; k < chunks; k++) {
+      QM_HIP(hipEventSynchronize(done[k & 1]));
+      if (k + 1 < chunks) issue(k + 1);
+      const size_t off = k * kStageChunk, n = std::min(kStageChunk, bytes - off);
+      std::memcpy(static_cast<char *>(dst) + off, stage[k & 1], n);
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipEventDestroy(done[0]);
+    (void)hipEventDestroy(done[1]);
+    throw;
+  }
+  QM_HIP(hipEventDestroy(done[0]));
+  QM_HIP(hipEventDestroy(done[1]));
+}
+
 void use_device(Ctx *c) {
   if (c == nullptr) fail(QM_ERR_ARG, "context is null");
   if (c->disposed) fail(QM_ERR_DISPOSED, "The HIP backend has been disposed.");
@@ -502,15 +578,14 @@ int qm_execute(qm_ctx *h, int32_t qubits, const qm_op *ops, int64_t n_ops, doubl
     std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
     State *s = state_new(c, qubits, QM_C128);
     try {
-      QM_HIP(hipMemcpyAsync(s->data, amps, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+      copy_to_device(c, s->data, amps, (size_t)bytes);
       if (profile) QM_HIP(hipEventRecord(c->ev_start, c->stream));
       ExecStats es = run_gates(s, gates, QM_SCHED_AUTO, cancel);
       if (profile) QM_HIP(hipEventRecord(c->ev_stop, c->stream));
       QM_HIP(hipStreamSynchronize(c->stream));
       check_cancel(cancel);
       // Commit only after the device work and the cancellation check (:166-176).
-      QM_HIP(hipMemcpyAsync(amps, s->data, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
-      QM_HIP(hipStreamSynchronize(c->stream));
+      copy_to_host(c, amps, s->data, (size_t)bytes);
       if (profile) {
         float ms = 0;
         QM_HIP(hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
@@ -582,12 +657,12 @@ int qm_state_upload(qm_state *h, const double *amps, int64_t n) {
     std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
     Ctx *c = s->ctx;
     if (s->precision == QM_C128) {
-      QM_HIP(hipMemcpyAsync(s->data, amps, (size_t)s->bytes, hipMemcpyHostToDevice, c->stream));
+      copy_to_device(c, s->data, amps, (size_t)s->bytes);
     } else {
       // Round through a temporary complex128 state on the device.
       State *tmp = state_new(c, s->qubits, QM_C128);
       try {
-        QM_HIP(hipMemcpyAsync(tmp->data, amps, (size_t)tmp->bytes, hipMemcpyHostToDevice, c->stream));
+        copy_to_device(c, tmp->data, amps, (size_t)tmp->bytes);
         launch_convert(tmp, s, c->stream);
         QM_HIP(hipStreamSynchronize(c->stream));
       } catch (...) {
@@ -607,12 +682,12 @@ int qm_state_download(qm_state *h, double *amps, int64_t n) {
     std::lock_guard<std::recursive_mutex> g(s->ctx->lock);
     Ctx *c = s->ctx;
     if (s->precision == QM_C128) {
-      QM_HIP(hipMemcpyAsync(amps, s->data, (size_t)s->bytes, hipMemcpyDeviceToHost, c->stream));
+      copy_to_host(c, amps, s->data, (size_t)s->bytes);
     } else {
       State *tmp = state_new(c, s->qubits, QM_C128);
       try {
         launch_convert(s, tmp, c->stream);
-        QM_HIP(hipMemcpyAsync(amps, tmp->data, (size_t)tmp->bytes, hipMemcpyDeviceToHost, c->stream));
+        copy_to_host(c, amps, tmp->data, (size_t)tmp->bytes);
         QM_HIP(hipStreamSynchronize(c->stream));
       } catch (...) {
         state_free(tmp);
@@ -1175,13 +1250,12 @@ int qm_noise_apply(qm_ctx *h, int32_t qubits, int32_t kind, double p, int32_t ta
       fail(QM_ERR_BUDGET, "HIP state exceeds the device or host staging memory budget.");
     State *s = state_new(c, qubits, QM_C128);
     try {
-      QM_HIP(hipMemcpyAsync(s->data, amps, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+      copy_to_device(c, s->data, amps, (size_t)bytes);
       noise_apply(s, kind, p, target, d);
       QM_HIP(hipStreamSynchronize(c->stream));
       check_cancel(cancel);
       // The caller's buffer is written only after the branch is applied.
-      QM_HIP(hipMemcpyAsync(amps, s->data, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
-      QM_HIP(hipStreamSynchronize(c->stream));
+      copy_to_host(c, amps, s->data, (size_t)bytes);
     } catch (...) {
       state_free(s);
       throw;

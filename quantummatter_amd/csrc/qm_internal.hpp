@@ -141,6 +141,10 @@ double *ctx_partials(Ctx *c, int64_t n);
 int64_t *ctx_ipartials(Ctx *c, int64_t n);
 double *ctx_pinned(Ctx *c, int64_t n);
 void *ctx_program(Ctx *c, int64_t bytes);
+// Synchronous host <-> device copies staged through two pinned chunks of the
+// context (the DMA of one chunk overlaps the host copy of the other).
+void copy_to_device(Ctx *c, void *dst, const void *src, size_t bytes);
+void copy_to_host(Ctx *c, void *dst, const void *src, size_t bytes);
 
 // ---- kernel launchers (kernels.hip) ---------------------------------------
 void launch_gate(State *s, const DevGate &g, hipStream_t st);

@@ -104,42 +104,6 @@ void expectation_values(const double *num, double mass, int64_t n, double *value
   }
 }
 
-// Device-to-host copy of a result through two pinned staging chunks of the
-// lane's context: the next chunk's DMA overlaps the host copy of the last.
-// Pinned staging runs the PCIe link several times faster than copies into
-// pageable caller memory.
-void download(Ctx *c, void *dst, const void *src, size_t bytes) {
-  constexpr size_t kChunkBytes = (size_t)32 << 20;
-  double *pin = ctx_pinned(c, (int64_t)(2 * kChunkBytes / sizeof(double)));
-  char *stage[2] = {reinterpret_cast<char *>(pin), reinterpret_cast<char *>(pin) + kChunkBytes};
-  hipEvent_t done[2];
-  QM_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
-  QM_HIP(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
-  const size_t chunks = (bytes + kChunkBytes - 1) / kChunkBytes;
-  auto issue = [&](size_t k) {
-    const size_t off = k * kChunkBytes, n = std::min(kChunkBytes, bytes - off);
-    QM_HIP(hipMemcpyAsync(stage[k & 1], static_cast<const char *>(src) + off, n, hipMemcpyDeviceToHost,
-                          c->stream));
-    QM_HIP(hipEventRecord(done[k & 1], c->stream));
-  };
-  try {
-    if (chunks) issue(0);
-    for (size_t k = 0; k < chunks; k++) {
-      QM_HIP(hipEventSynchronize(done[k & 1]));
-      if (k + 1 < chunks) issue(k + 1);
-      const size_t off = k * kChunkBytes, n = std::min(kChunkBytes, bytes - off);
-      std::memcpy(static_cast<char *>(dst) + off, stage[k & 1], n);
-    }
-  } catch (...) {
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipEventDestroy(done[0]);
-    (void)hipEventDestroy(done[1]);
-    throw;
-  }
-  QM_HIP(hipEventDestroy(done[0]));
-  QM_HIP(hipEventDestroy(done[1]));
-}
-
 // One request on a full-chip lane: resident state, fused passes, result.
 void run_single(Ctx *c, Req *r, int schedule) {
   State *s = state_new(c, r->qubits, r->precision);
@@ -147,7 +111,7 @@ void run_single(Ctx *c, Req *r, int schedule) {
     launch_set_basis(s, 0, c->stream);
     run_gates(s, r->gates, schedule, nullptr);
     if (r->kind == QM_REQ_AMPLITUDES) {
-      download(c, r->out, s->data, (size_t)s->bytes);
+      copy_to_host(c, r->out, s->data, (size_t)s->bytes);
     } else if (r->kind == QM_REQ_SHOTS) {
       r->fallbacks = cdf_search(s, r->uniforms.data(), (int64_t)r->uniforms.size(),
                                 static_cast<int64_t *>(r->out), c->stream);
@@ -219,7 +183,7 @@ void run_batched(Ctx *c, const std::vector<Req *> &group, const FusedShape &shap
   L.on_states = [&](const void *states, const double *, hipStream_t) {
     for (size_t i = 0; i < group.size(); i++)
       if (group[i]->kind == QM_REQ_AMPLITUDES)
-        download(c, group[i]->out, static_cast<const char *>(states) + L.items[i].state_off * ab,
+        copy_to_host(c, group[i]->out, static_cast<const char *>(states) + L.items[i].state_off * ab,
                  (size_t)(L.items[i].len * ab));
   };
   run_batch_launch(c, L);
