@@ -190,14 +190,19 @@ std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
 
 // Tuning knobs read from the environment (unset: no change): QM_CONTIGUOUS
 // (lowest index bits kept in every tile), QM_REMAP (remapping mode 0..2),
-// QM_SEARCH (tile search 0/1), QM_MAX_PHASES (phases per pass at most).
+// QM_SEARCH (tile search 0/1), QM_MAX_PHASES (phases per pass at most),
+// QM_PHASE_SEARCH (register sets of the phases 0..2) and QM_PHASE_LOW_PENALTY
+// (charge, in gates, of a contiguous low bit in a pass's first register set).
 void apply_tuning_knobs(FusedShape &shape) {
   auto knob = [](const char *name) {
     const char *e = std::getenv(name);
     return e ? std::atoi(e) : -1;
   };
   static const int contiguous = knob("QM_CONTIGUOUS"), remap = knob("QM_REMAP"), search = knob("QM_SEARCH"),
-                   max_phases = knob("QM_MAX_PHASES");
+                   max_phases = knob("QM_MAX_PHASES"), phase_search = knob("QM_PHASE_SEARCH"),
+                   low_penalty = knob("QM_PHASE_LOW_PENALTY");
+  if (phase_search >= 0 && phase_search <= 2) shape.phase_search = phase_search;
+  if (low_penalty >= 0) shape.low_penalty = low_penalty;
   if (contiguous >= 0 && contiguous < shape.K) shape.contiguous = contiguous, shape.alt_contiguous = 0;
   if (remap >= 0 && remap <= 2) shape.remap = remap;
   if (search == 0 || search == 1) shape.search = search;

@@ -254,7 +254,14 @@ std::string hex(uint64_t v) {
 // bit 11 (with bit 7) reads the coefficients through the constant address
 // space, so they are scalar loads rather than vector loads whose waits also
 // wait for the state stores in flight (2 % faster in complex128, 5 % in
-// complex64).
+// complex64), bit 12 (with bits 5 and 10) carries the flips of bit 5 through
+// X gates controlled by one flipped register bit (the register permutation
+// plus the control's flag) and folds the flips left at an exchange or at the
+// store into the LDS write or store addresses instead of swapping registers
+// under a branch, with each exchange's layout chosen for the folded writer
+// addresses and a per-thread factor for flipped diagonals (the moves of the
+// conditional swaps go, 7 % fewer VALU instructions, but 2 % slower in
+// complex128 and 5 % in complex64: more spills and waits).
 int jit_variant(int precision) {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
@@ -309,8 +316,13 @@ static int gf2_rank(std::vector<uint32_t> v) {
 // place and gives every position a B-bit low part: positions below B a basis
 // of the low bits, the others any value (block triangular, so invertible).
 // Values are found by a seeded randomised search; identity if none is found.
+// Folded flips (position, thread-bit mask): a writer whose thread bits in the
+// mask have odd parity writes the amplitude at that position flipped, so the
+// image of writer thread bit k also carries the positions of the folds whose
+// mask holds k.
 static std::vector<uint32_t> exchange_layout(const DevPass &d, int from, int to, int K, int L,
-                                             int elem_bytes) {
+                                             int elem_bytes,
+                                             const std::vector<std::pair<int, uint32_t>> &folds = {}) {
   int b = 0;
   while ((elem_bytes << (b + 1)) <= 128) b++;
   b = std::min({b, L - 1, K - 1});
@@ -319,18 +331,25 @@ static std::vector<uint32_t> exchange_layout(const DevPass &d, int from, int to,
   for (int p = 0; p < K; p++) cols[p] = 1u << p;
   if (b < 1) return cols;
   const uint32_t low = (1u << B) - 1;
-  // Groups: (positions, bits that must be independent).
-  std::vector<std::pair<std::vector<int>, int>> groups;
+  // Groups: (images as sets of positions XORed, bits that must be independent).
+  std::vector<std::pair<std::vector<std::vector<int>>, int>> groups;
   for (int slot : {from, to})
     for (int n : {b, B}) {
-      std::vector<int> g;
-      for (int k = 0; k < n; k++) g.push_back(d.lanepos[slot][k]);
+      std::vector<std::vector<int>> g;
+      for (int k = 0; k < n; k++) {
+        std::vector<int> e{d.lanepos[slot][k]};
+        if (slot == from)
+          for (const auto &f : folds)
+            if ((f.second >> k) & 1) e.push_back(f.first);
+        g.push_back(e);
+      }
       groups.push_back({g, n});
     }
   std::vector<int> order;
   for (auto &g : groups)
-    for (int p : g.first)
-      if (std::find(order.begin(), order.end(), p) == order.end()) order.push_back(p);
+    for (const auto &e : g.first)
+      for (int p : e)
+        if (std::find(order.begin(), order.end(), p) == order.end()) order.push_back(p);
   uint64_t rng = 0x9E3779B97F4A7C15ull ^ ((uint64_t)from << 32) ^ (uint64_t)to;
   auto next = [&] {
     rng ^= rng << 13;
@@ -343,8 +362,15 @@ static std::vector<uint32_t> exchange_layout(const DevPass &d, int from, int to,
   auto ok = [&] {
     for (auto &g : groups) {
       std::vector<uint32_t> v;
-      for (int p : g.first)
-        if (set[p]) v.push_back(val[p] & ((1u << g.second) - 1));
+      for (const auto &e : g.first) {
+        bool all = true;
+        uint32_t x = 0;
+        for (int p : e) {
+          all = all && set[p];
+          x ^= val[p];
+        }
+        if (all) v.push_back(x & ((1u << g.second) - 1));
+      }
       if (gf2_rank(v) != (int)v.size()) return false;
     }
     std::vector<uint32_t> basis;  // low parts of positions below B stay independent
@@ -400,6 +426,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const bool launder = (variant & 256) != 0, fold = (variant & 512) != 0;
   const bool banked = (variant & 1024) != 0 && split;
   const bool scalar_coef = (variant & 2048) != 0 && tile_coef;
+  const bool fold_frames = (variant & 4096) != 0 && frames && banked;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -502,7 +529,14 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   } else {
     loads("vr", "vi", "base");
   }
-  auto transpose = [&](int from, int to) {
+  // A flip folded into an exchange: its position and, when its condition is
+  // linear, the thread bits whose parity it is.
+  struct Fold {
+    int lp;
+    uint32_t tmask;
+    bool linear;
+  };
+  auto transpose = [&](int from, int to, const std::vector<Fold> &folds) {
     o << "    {\n";
     if (inline_j) {
       o << "    const unsigned j" << from << " = " << lane_addr(from) << ";\n";
@@ -516,8 +550,10 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     if (banked) {
       // Per-exchange layout (exchange_layout): lane parts formed here from a
       // laundered thread index, register parts as constants.
-      const std::vector<uint32_t> cols =
-          exchange_layout(d, from, to, sh.K, L, c128 ? 8 : 4);
+      std::vector<std::pair<int, uint32_t>> fl;
+      for (const Fold &f : folds)
+        if (f.linear) fl.push_back({f.lp, f.tmask});
+      const std::vector<uint32_t> cols = exchange_layout(d, from, to, sh.K, L, c128 ? 8 : 4, fl);
       auto S = [&](uint32_t x) {
         uint32_t y = 0;
         for (int q = 0; q < sh.K; q++)
@@ -533,14 +569,22 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       };
       // With fold, the register constant's bits outside every lane image are
       // added (immediate offset) instead of XORed, as in `at` below.
+      // Folded frames: the writer XORs its lane part with the image of each
+      // flipped position, so those bits are XOR parts of the write addresses.
+      uint32_t fmask = 0;
+      std::string fterm;
+      for (const Fold &f : folds) {
+        fmask |= S(1u << f.lp);
+        fterm += " ^ (fq" + std::to_string(f.lp) + " ? " + std::to_string(S(1u << f.lp)) + "u : 0u)";
+      }
       auto addr = [&](const char *j, int slot, uint32_t c) {
         if (!fold) return std::string("lds[") + j + " ^ " + std::to_string(c) + "u]";
-        uint32_t m = 0;
+        uint32_t m = slot == from ? fmask : 0u;
         for (int k = 0; k < L; k++) m |= S(1u << d.lanepos[slot][k]);
         return std::string("lds[(") + j + " ^ " + std::to_string(c & m) + "u) + " + std::to_string(c & ~m) + "u]";
       };
       o << "    unsigned tt = t;\n    asm volatile(\"\" : \"+v\"(tt));\n";
-      o << "    const unsigned jw = " << lane_expr(from) << ", jr = " << lane_expr(to) << ";\n";
+      o << "    const unsigned jw = " << lane_expr(from) << fterm << ", jr = " << lane_expr(to) << ";\n";
       for (const char *part : {"vr", "vi"}) {
         o << "    __syncthreads();\n";
         for (int r = 0; r < N; r++)
@@ -615,6 +659,8 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   struct Frame {
     bool lane = false;                     // depends on thread bits
     std::vector<std::pair<int, int>> deps;  // (tile position, thread bit)
+    uint32_t tmask = 0;                    // the condition's thread bits (XOR), when linear
+    bool nonlinear = false;                // a product of thread bits
   };
   std::vector<char> framed((size_t)K, 0);
   std::vector<Frame> fdep((size_t)K);
@@ -627,6 +673,12 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       const TileOp &op = ops[d.phase_op_begin[ph] + i];
       if (is_source(op)) declared[(size_t)lp_reg(ph, op.rbit)] = 1;
     }
+  if (fold_frames)
+    for (int ph = 0; ph < d.nphases; ph++)
+      for (int i = 0; i < d.phase_op_count[ph]; i++) {
+        const TileOp &op = ops[d.phase_op_begin[ph] + i];
+        if (op.kind == G_X && op.rbit >= 0) declared[(size_t)lp_reg(ph, op.rbit)] = 1;
+      }
   for (int lp = 0; lp < K; lp++)
     if (declared[(size_t)lp]) o << "    bool " << fname(lp) << " = false;\n";
   auto cond = [&](const TileOp &op, int slot) {
@@ -671,14 +723,27 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   };
   // LDS transpose, first performing the exchanges whose flags would not
   // survive it.
+  // With fold_frames, such an exchange is instead folded into the LDS write
+  // addresses: the amplitude a thread holds at a flipped position is written
+  // where it belongs, so no register moves.
   auto move_to = [&](int from, int to) {
+    std::vector<Fold> folds;
     for (int lp = 0; lp < K; lp++) {
       if (!framed[(size_t)lp] || !fdep[(size_t)lp].lane) continue;
       bool keep = reg_bit(to, lp) >= 0;
       for (const auto &dk : fdep[(size_t)lp].deps) keep = keep && d.lanepos[to][dk.second] == dk.first;
-      if (!keep) settle(from, reg_bit(from, lp));
+      if (keep) continue;
+      if (fold_frames)
+        folds.push_back({lp, fdep[(size_t)lp].tmask, !fdep[(size_t)lp].nonlinear});
+      else
+        settle(from, reg_bit(from, lp));
     }
-    transpose(from, to);
+    transpose(from, to, folds);
+    for (const Fold &f : folds) {
+      o << "    " << fname(f.lp) << " = false;\n";
+      framed[(size_t)f.lp] = 0;
+      fdep[(size_t)f.lp] = Frame{};
+    }
   };
   auto sel = [&](int lp, const std::string &when_set, const std::string &otherwise) {
     return framed[(size_t)lp] ? "(" + fname(lp) + " ? " + when_set + " : " + otherwise + ")" : otherwise;
@@ -708,21 +773,64 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   }
   if (use_gneg) o << "    bool gneg = false;\n";
   auto cf_str = [](size_t k) { return "(F)cf[" + std::to_string(k) + "]"; };
+  // Per-thread factor of a phase (fold_frames, phases that end with a
+  // multiplication of every register anyway: merged diagonals or the pass's
+  // factors): a factored diagonal on a flipped position multiplies the other
+  // half by the inverse ratio and the thread's factor by the ratio, and a
+  // factored Ry in form b on a flipped position negates the thread's factor,
+  // instead of multiplying both halves.
   for (int ph = 0; ph < d.nphases; ph++) {
     if (cur != ph) move_to(cur, ph);
     cur = ph;
+    const bool gl_ok = fold_frames && (d.phase_diag_count[ph] > 0 || ph == d.nphases - 1);
+    const std::string glr = "glr" + std::to_string(ph), gli = "gli" + std::to_string(ph);
+    bool gl_used = false;
+    if (gl_ok) o << "    F " << glr << " = (F)1, " << gli << " = (F)0;\n";
     for (int i = 0; i < d.phase_op_count[ph]; i++) {
       const TileOp &op = ops[d.phase_op_begin[ph] + i];
+      // An X on register bit b controlled by one register bit c whose
+      // position is flipped under condition f: X_b^(r_c xor f) is the
+      // register permutation X_b^(r_c) followed by the flip X_b^f, so the
+      // permutation is done on the registers and f is added to b's flag.
+      if (fold_frames && op.kind == G_X && op.rbit >= 0 && __builtin_popcount(op.reg_cmask) == 1 &&
+          !op.lane_cmask && !op.out_cmask && framed[(size_t)lp_reg(ph, __builtin_ctz(op.reg_cmask))]) {
+        const int clp = lp_reg(ph, __builtin_ctz(op.reg_cmask)), xlp = lp_reg(ph, op.rbit);
+        o << "    { qj_x" << tmpl << op.rbit << ", " << op.reg_cmask << ">(vr, vi); }\n";
+        o << "    " << fname(xlp) << " ^= " << fname(clp) << ";\n";
+        const Frame g = fdep[(size_t)clp];
+        Frame &f = fdep[(size_t)xlp];
+        f.lane = f.lane || g.lane;
+        f.deps.insert(f.deps.end(), g.deps.begin(), g.deps.end());
+        f.tmask ^= g.tmask;
+        f.nonlinear = f.nonlinear || g.nonlinear;
+        framed[(size_t)xlp] = 1;
+        continue;
+      }
       for (int b = 0; b < R; b++)
         if (op.reg_cmask & (1u << b)) settle(ph, b);
       const int tlp = op.rbit >= 0 ? lp_reg(ph, op.rbit) : (op.dreg ? lp_reg(ph, __builtin_ctz(op.dreg)) : -1);
       if (op.rbit >= 0 && (op.kind == G_H || op.kind == G_Y || op.kind == G_MAT)) settle(ph, op.rbit);
+      // A factored Ry in form b on a position flipped under thread-bit
+      // conditions: its factor's sign would depend on the thread's data, so
+      // the flip is performed first, or (fold_frames) the Ry is applied
+      // unfactored with its sine's sign chosen per thread.
+      bool unfactor = false;
       if (factor_rot(op) && op.kind == G_RY && framed[(size_t)tlp] && fdep[(size_t)tlp].lane &&
-          std::fabs(op.c[0]) < std::fabs(op.c[1]))
-        settle(ph, op.rbit);  // its factor's sign would depend on the thread's data
+          std::fabs(op.c[0]) < std::fabs(op.c[1])) {
+        if (gl_ok)
+          ;
+        else if (fold_frames)
+          unfactor = true;
+        else
+          settle(ph, op.rbit);
+      }
       if (is_source(op)) {
         o << "    " << fname(tlp) << " ^= (" << cond(op, ph) << ");\n";
         Frame &f = fdep[(size_t)tlp];
+        if (__builtin_popcount(op.lane_cmask) == 1 && !framed[(size_t)d.lanepos[ph][__builtin_ctz(op.lane_cmask)]])
+          f.tmask ^= op.lane_cmask;
+        else if (op.lane_cmask)
+          f.nonlinear = true;
         for (int k = 0; k < L; k++)
           if (op.lane_cmask & (1u << k)) {
             const int c = d.lanepos[ph][k];
@@ -739,7 +847,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       const std::string c = cond(op, ph);
       o << "    " << (c.empty() ? "{" : "if (" + c + ") {") << " ";
       const std::string m = std::to_string(op.reg_cmask);
-      if (factor_rot(op)) {
+      if (factor_rot(op) && !unfactor) {
         const double c = op.c[0], sn = op.c[1];
         const bool form_a = std::fabs(c) >= std::fabs(sn);
         const double ratio[1] = {form_a ? sn / c : c / sn};
@@ -752,7 +860,14 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
           // X Ry(theta) X = Ry(-theta): the ratio changes sign, and in form b
           // so does the factor taken out.
           arg = sel(tlp, "-" + cf_str(k), cf_str(k));
-          if (!form_a && framed[(size_t)tlp]) o << "gneg ^= " << fname(tlp) << "; ";
+          if (!form_a && framed[(size_t)tlp] && fdep[(size_t)tlp].lane) {
+            const std::string f = fname(tlp);
+            o << glr << " = " << f << " ? -" << glr << " : " << glr << "; " << gli << " = " << f << " ? -" << gli
+              << " : " << gli << "; ";
+            gl_used = true;
+          } else if (!form_a && framed[(size_t)tlp]) {
+            o << "gneg ^= " << fname(tlp) << "; ";
+          }
         }
         o << (op.kind == G_RX ? "qj_rx" : "qj_ry") << (form_a ? "a" : "b") << tmpl << op.rbit << ">(vr, vi, " << arg
           << ");";
@@ -767,7 +882,18 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
         gim = ni;
         const size_t k = cfix(q, 2);
         const int b = __builtin_ctz(op.dreg);
-        if (framed[(size_t)tlp])
+        const double dden = dr * dr + di * di;
+        if (framed[(size_t)tlp] && gl_ok && dden > 0) {
+          const double qi[2] = {(ar * dr + ai * di) / dden, (ai * dr - ar * di) / dden};
+          const size_t k2 = cfix(qi, 2);
+          const std::string f = fname(tlp);
+          o << "qj_diag" << tmpl << b << ", 0, false>(vr, vi, (F)1, (F)0, (" << f << " ? " << cf_str(k2) << " : "
+            << cf_str(k) << "), (" << f << " ? " << cf_str(k2 + 1) << " : " << cf_str(k + 1) << ")); ";
+          o << "const F fr = " << f << " ? " << cf_str(k) << " : (F)1, fi = " << f << " ? " << cf_str(k + 1)
+            << " : (F)0; const F nr = " << glr << " * fr - " << gli << " * fi, ni = " << glr << " * fi + " << gli
+            << " * fr; " << glr << " = nr; " << gli << " = ni;";
+          gl_used = true;
+        } else if (framed[(size_t)tlp])
           o << "qj_diag" << tmpl << b << ", 0, true>(vr, vi, " << sel(tlp, cf_str(k), "(F)1") << ", "
             << sel(tlp, cf_str(k + 1), "(F)0") << ", " << sel(tlp, "(F)1", cf_str(k)) << ", "
             << sel(tlp, "(F)0", cf_str(k + 1)) << ");";
@@ -836,12 +962,18 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       g0 = cf_str(gslot), g1 = cf_str(gslot + 1);
       if (use_gneg) g0 = "(gneg ? -" + g0 + " : " + g0 + ")", g1 = "(gneg ? -" + g1 + " : " + g1 + ")";
     }
-    if (!dcnt && any_factor && last)
+    if (!dcnt && any_factor && last && !gl_used)
       o << "    qj_phase" << tmpl << "0>(vr, vi, " << g0 << ", " << g1 << ");\n";
+    if (!dcnt && gl_used)
+      o << "    { const F pr = " << g0 << " * " << glr << " - " << g1 << " * " << gli << ", pi = " << g0 << " * " << gli
+        << " + " << g1 << " * " << glr << "; qj_phase" << tmpl << "0>(vr, vi, pr, pi); }\n";
     if (dcnt) {
       // Merged lane/outside diagonals: one per-lane phase, one multiplication
       // (the last phase's starts from the factors taken out of the pass).
       o << "    { F pr = " << g0 << ", pi = " << g1 << ";\n";
+      if (gl_used)
+        o << "      { const F nr = pr * " << glr << " - pi * " << gli << ", ni = pr * " << gli << " + pi * " << glr
+          << "; pr = nr; pi = ni; }\n";
       for (int i = 0; i < dcnt; i++) {
         const TileOp &q = ops[d.phase_diag_begin[ph] + i];
         const size_t k = cfix(q.c, 8);
@@ -865,8 +997,12 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   if (!d.store_direct) move_to(cur, kIoStore);
   // Pending exchanges at the store: a thread bit flips the store address, a
   // register bit is performed on the registers.
+  // With fold_frames a flipped register bit b moves the address instead: the
+  // registers with bit b clear store that many amplitudes higher, those with
+  // it set lower (one base address per combination of flipped bits).
   const int st = d.store_direct ? cur : kIoStore;
   std::string flips;
+  std::vector<std::pair<int, int>> rflips;  // (register bit, position)
   for (int lp = 0; lp < K; lp++) {
     if (!framed[(size_t)lp]) continue;
     int lane = -1;
@@ -876,16 +1012,37 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       flips += " ^ (" + fname(lp) + " ? " + hex(1ull << d.store_lane_gbit[lane]) + " : 0ull)";
     } else {
       for (int b = 0; b < R; b++)
-        if (lp_reg(st, b) == lp) settle(st, b);
+        if (lp_reg(st, b) == lp) {
+          if (fold_frames)
+            rflips.push_back({b, lp});
+          else
+            settle(st, b);
+        }
     }
   }
   o << "    { " << T << " *dst = s + (base | (sl" << flips << "));\n";
+  const int nf = (int)rflips.size();
+  for (int m = 0; nf && m < (1 << nf); m++) {
+    o << "      " << T << " *const dst" << m << " = dst";
+    for (int i = 0; i < nf; i++)
+      o << " + (" << fname(rflips[(size_t)i].second) << " ? " << (((m >> i) & 1) ? "-" : "") << "(long long)"
+        << hex(d.store_reg_off[1u << rflips[(size_t)i].first]) << " : 0ll)";
+    o << ";\n";
+  }
+  auto dst_of = [&](int r) {
+    if (!nf) return std::string("dst");
+    int m = 0;
+    for (int i = 0; i < nf; i++)
+      if ((r >> rflips[(size_t)i].first) & 1) m |= 1 << i;
+    return "dst" + std::to_string(m);
+  };
   for (int r = 0; r < N; r++) {
     if (nontemporal)
-      o << "      { " << T << " *a = dst + " << hex(d.store_reg_off[r]) << "; __builtin_nontemporal_store(vr[" << r
-        << "], &a->re); __builtin_nontemporal_store(vi[" << r << "], &a->im); }\n";
+      o << "      { " << T << " *a = " << dst_of(r) << " + " << hex(d.store_reg_off[r])
+        << "; __builtin_nontemporal_store(vr[" << r << "], &a->re); __builtin_nontemporal_store(vi[" << r
+        << "], &a->im); }\n";
     else
-      o << "      { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; dst["
+      o << "      { " << T << " a; a.re = vr[" << r << "]; a.im = vi[" << r << "]; " << dst_of(r) << "["
         << hex(d.store_reg_off[r]) << "] = a; }\n";
   }
   o << "    }\n  }\n}\n";

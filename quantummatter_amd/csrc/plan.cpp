@@ -494,6 +494,67 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         }
         scan_phase(rset, false);
       }
+      if (sh.phase_search && !next.empty()) {
+        // Phase search: grow a register set position by position for the most
+        // gates runnable in this phase (same blocking rule; ties: the position
+        // a pending gate needs first), optionally improve it by single
+        // exchanges, and keep it if it runs more gates than the in-order set.
+        const int penalty = rsets.empty() ? sh.low_penalty : 0;
+        auto score = [&](uint32_t rs) {
+          uint64_t pb = 0;
+          int cnt = (rs & low_local) ? -penalty : 0;
+          for (int idx : remaining) {
+            const PGate &x = G[idx];
+            if ((x.touch & pb) || (x.pair && !(rs & (1u << local_of[x.g.target])))) {
+              pb |= x.touch;
+              continue;
+            }
+            cnt++;
+          }
+          return cnt;
+        };
+        int first_need[32];
+        for (int i = 0; i < 32; i++) first_need[i] = INT32_MAX;
+        for (size_t k = 0; k < remaining.size(); k++) {
+          const PGate &x = G[remaining[k]];
+          const int lp = local_of[x.g.target];
+          if (x.pair && first_need[lp] == INT32_MAX) first_need[lp] = (int)k;
+        }
+        uint32_t S = 0;
+        while (popc(S) < R) {
+          int best = -1, best_score = INT32_MIN;
+          for (int q = 0; q < K; q++) {
+            if (S & (1u << q)) continue;
+            const int v = score(S | (1u << q));
+            if (v > best_score || (v == best_score && first_need[q] < first_need[best])) {
+              best = q;
+              best_score = v;
+            }
+          }
+          S |= 1u << best;
+        }
+        int cur_score = score(S);
+        for (bool improved = sh.phase_search >= 2; improved;) {
+          improved = false;
+          for (int a = 0; a < K && !improved; a++) {
+            if (!(S & (1u << a))) continue;
+            for (int b = 0; b < K && !improved; b++) {
+              if (S & (1u << b)) continue;
+              const uint32_t T = (S & ~(1u << a)) | (1u << b);
+              const int v = score(T);
+              if (v > cur_score) {
+                S = T;
+                cur_score = v;
+                improved = true;
+              }
+            }
+          }
+        }
+        if (cur_score > (int)here.size() - ((rset & low_local) ? penalty : 0)) {
+          rset = S;
+          scan_phase(rset, false);
+        }
+      }
       // Fill the register set from the top local positions (keeps the
       // contiguous low bits on lanes, so loads and stores stay coalesced).
       for (int p = K - 1; popc(rset) < R && p >= 0; p--)

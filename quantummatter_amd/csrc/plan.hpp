@@ -197,6 +197,15 @@ struct FusedShape {
   // The greedy tile search finds fewer passes with one or the other value
   // circuit by circuit.
   int alt_contiguous = 0;
+  // Register sets of the phases: 0 in gate order (the targets of the first
+  // pending gates); 1 also grown one register position at a time by the
+  // position that lets the most pending gates of the pass run in the phase,
+  // the better of the two taken; 2 as 1, then improved by exchanging one
+  // position for another while that lets more gates run.
+  int phase_search = 2;
+  // With phase_search: gates a first phase's register set is charged for
+  // holding a contiguous low bit (the load then needs a transpose).
+  int low_penalty = 0;
 };
 
 struct FusedPass {
