@@ -3,13 +3,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <exception>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "dense.hpp"
@@ -209,6 +211,76 @@ void apply_tuning_knobs(FusedShape &shape) {
   if (max_phases >= 1) shape.max_phases = max_phases, shape.alt_contiguous = 0;
 }
 
+// Plans by gate structure.  The planner's decisions (each pass's tile, each
+// phase's register set, and for the compiled schedule the chosen candidate)
+// depend on the gates' kinds, targets and controls only, so they are kept per
+// structure, width, precision and shape, and a circuit of a known structure
+// (new angles: a parameter sweep, a VQE loop, a repeated served circuit) is
+// planned by replaying them instead of searching again.  At most kPlanCacheMax
+// structures are kept, the oldest dropped first.
+namespace {
+constexpr size_t kPlanCacheMax = 256;
+struct CachedPlan {
+  FusedShape shape;
+  PlanTrace trace;
+};
+struct PlanCache {
+  std::mutex mu;
+  std::unordered_map<std::string, CachedPlan> map;
+  std::deque<std::string> order;
+};
+PlanCache &plan_cache() {
+  static PlanCache *c = new PlanCache;  // outlives every context
+  return *c;
+}
+std::string plan_key(const std::vector<DevGate> &gates, int n, int precision, const FusedShape &sh,
+                     int schedule) {
+  std::string k;
+  auto put = [&](const void *p, size_t bytes) { k.append((const char *)p, bytes); };
+  const int32_t head[] = {n,           precision,      schedule,         sh.K,         sh.L,
+                          sh.R,        sh.contiguous,  sh.remap,         sh.search,    sh.max_phases,
+                          sh.alt_contiguous, sh.phase_search, sh.low_penalty};
+  put(head, sizeof head);
+  k.reserve(k.size() + gates.size() * 28);
+  for (const DevGate &g : gates) {
+    const int32_t v[] = {g.kind, g.target, g.swap_other, g.aux0, g.aux1};
+    put(v, sizeof v);
+    put(&g.ctrl_mask, sizeof g.ctrl_mask);
+  }
+  return k;
+}
+bool plan_cache_get(const std::string &key, CachedPlan *out) {
+  PlanCache &c = plan_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.map.find(key);
+  if (it == c.map.end()) return false;
+  *out = it->second;
+  return true;
+}
+void plan_cache_put(const std::string &key, CachedPlan v) {
+  PlanCache &c = plan_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (c.map.count(key)) return;
+  while (c.order.size() >= kPlanCacheMax) {
+    c.map.erase(c.order.front());
+    c.order.pop_front();
+  }
+  c.order.push_back(key);
+  c.map.emplace(key, std::move(v));
+}
+// plan_fused through the cache: replays the structure's decisions when known.
+FusedPlan plan_cached(const std::vector<DevGate> &gates, int n, int precision, const FusedShape &shape,
+                      int schedule) {
+  const std::string key = plan_key(gates, n, precision, shape, schedule);
+  CachedPlan hit;
+  if (plan_cache_get(key, &hit)) return plan_fused(gates, n, hit.shape, &hit.trace);
+  CachedPlan rec{shape, {}};
+  FusedPlan plan = plan_fused(gates, n, shape, nullptr, &rec.trace);
+  plan_cache_put(key, std::move(rec));
+  return plan;
+}
+}  // namespace
+
 // Runs prepared gates on a resident state with the chosen schedule.
 // Whether QM_SCHED_AUTO compiles a structure's kernels in the background on
 // its first (interpreted) execution; tuning knob QM_AUTO_COMPILE=0 turns it off.
@@ -263,17 +335,24 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
   int schedule = automatic ? QM_SCHED_COMPILED : schedule_in;
   FusedShape shape = shape_for(schedule);
   if (schedule == QM_SCHED_COMPILED && s->qubits >= shape.K && gates.size() > 1) {
-    FusedPlan plan = plan_fused(gates, s->qubits, shape);
-    if (shape.alt_contiguous > 0 && shape.alt_contiguous < shape.K && shape.alt_contiguous != shape.contiguous) {
+    const std::string key = plan_key(gates, s->qubits, s->precision, shape, QM_SCHED_COMPILED);
+    CachedPlan hit;
+    const bool known = plan_cache_get(key, &hit);
+    CachedPlan rec{shape, {}};
+    FusedPlan plan = known ? plan_fused(gates, s->qubits, hit.shape, &hit.trace)
+                           : plan_fused(gates, s->qubits, shape, nullptr, &rec.trace);
+    if (!known && shape.alt_contiguous > 0 && shape.alt_contiguous < shape.K &&
+        shape.alt_contiguous != shape.contiguous) {
       // Candidate plans: both low-bit counts, each with the full phase budget
       // and with at most 9 phases per pass; the cheapest under the pass-time
-      // model runs.  Model, fitted on the 30-qubit c128 layer circuit (14
-      // passes, 3 to 11 exchanges each): 1.6 ms per pass + 2.0 ms per LDS
-      // exchange over a 16 GiB state; it ranks that circuit's plans (c = 3, 4;
-      // 9 to 12 phases) as measured, within 3 %.
+      // model runs.  Model, fitted on four plans of the 30-qubit c128 layer
+      // circuit with searched phases (14 to 19 passes, 72 to 101 exchanges;
+      // 54 ms + 5.1 ms per pass + 0.87 ms per LDS exchange over a 16 GiB
+      // state, the constant being the gates' arithmetic, the same in every
+      // plan; within 1 % of the four measurements).
       auto cost = [](const FusedPlan &p) {
         double t = 0;
-        for (const FusedPass &x : p.passes) t += 1.6 + 2.0 * x.transposes;
+        for (const FusedPass &x : p.passes) t += 5.1 + 0.87 * x.transposes;
         return t;
       };
       double best = cost(plan);
@@ -283,14 +362,17 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
           FusedShape alt = shape;
           alt.contiguous = low;
           alt.max_phases = phases;
-          FusedPlan other = plan_fused(gates, s->qubits, alt);
+          PlanTrace trace;
+          FusedPlan other = plan_fused(gates, s->qubits, alt, nullptr, &trace);
           const double c = cost(other);
           if (c < best) {
             best = c;
             plan = std::move(other);
+            rec = CachedPlan{alt, std::move(trace)};
           }
         }
     }
+    if (!known) plan_cache_put(key, std::move(rec));
     JitProgram jp;
     bool ready = false;
     if (jit_supported(plan)) {
@@ -330,7 +412,7 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
     shape.search = search;
   }
   if (schedule == QM_SCHED_FUSED && s->qubits >= shape.K && gates.size() > 1) {
-    FusedPlan plan = plan_fused(gates, s->qubits, shape);
+    FusedPlan plan = plan_cached(gates, s->qubits, s->precision, shape, QM_SCHED_FUSED);
     std::vector<TileOp> flat = plan.flat_ops();
     const int64_t pass_bytes = (int64_t)(plan.passes.size() * sizeof(DevPass));
     const int64_t op_bytes = (int64_t)(flat.size() * sizeof(TileOp));
@@ -1122,7 +1204,7 @@ int qm_autotune(qm_ctx *h, int32_t qubits, int32_t precision, int32_t schedule, 
         if (K > qubits || R < 2 || R > 5 || K < 12 - (precision == QM_C128))
           continue;
         cands.push_back(compiled_shape(K, L));
-        if (K == 14 && L == 9)  // the default shape with 3 and 5 low bits kept in the tiles
+        if ((K == 14 && L == 9) || (K == 13 && L == 8))  // the defaults with 3 and 5 low bits kept
           for (int low : {3, 5}) cands.push_back(compiled_shape(K, L, low));
         if (K == 13 && L == 8) cands.push_back(FusedShape{K, L, R, contiguous});  // without remapping
       }
@@ -1197,12 +1279,33 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
     FusedShape shape{tile_bits, lane_bits, tile_bits - lane_bits, contiguous};
     shape.remap = flags & 3;
     shape.search = (flags >> 2) & 1;
-    if (flags < 0 || flags > 7 || shape.remap == 3) fail(QM_ERR_RANGE, "unsupported planning flags");
+    if (flags < 0 || flags > 15 || shape.remap == 3) fail(QM_ERR_RANGE, "unsupported planning flags");
     if (shape.R < 1 || shape.R > 5 || shape.L < 1 || shape.L > 10 || shape.K > kMaxK ||
         contiguous < 0 || contiguous >= shape.K)
       fail(QM_ERR_RANGE, "unsupported tile shape");
     if (qubits < shape.K) {
       for (const DevGate &x : g) apply_gate_host(x, amps, qubits);
+      return;
+    }
+    if (flags & 8) {
+      // Replay: decisions recorded on the same structure with other
+      // coefficients, then replayed on these gates; the replayed plan must
+      // equal a fresh plan of these gates in every table and op.
+      std::vector<DevGate> other = g;
+      for (DevGate &x : other)
+        for (double &v : x.c) v = 0.5 * v + 0.25;
+      PlanTrace trace;
+      (void)plan_fused(other, qubits, shape, nullptr, &trace);
+      FusedPlan replayed = plan_fused(g, qubits, shape, &trace);
+      FusedPlan fresh = plan_fused(g, qubits, shape);
+      bool same = replayed.passes.size() == fresh.passes.size();
+      for (size_t i = 0; same && i < fresh.passes.size(); i++) {
+        const FusedPass &a = replayed.passes[i], &b = fresh.passes[i];
+        same = std::memcmp(&a.dev, &b.dev, sizeof(DevPass)) == 0 && a.ops.size() == b.ops.size() &&
+               (a.ops.empty() || std::memcmp(a.ops.data(), b.ops.data(), a.ops.size() * sizeof(TileOp)) == 0);
+      }
+      if (!same) fail(QM_ERR_DEVICE, "a replayed plan differs from the fresh plan");
+      apply_plan_host(replayed, amps, qubits);
       return;
     }
     FusedPlan plan = plan_fused(g, qubits, shape);

@@ -279,7 +279,8 @@ std::vector<TileOp> FusedPlan::flat_ops() {
   return all;
 }
 
-FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedShape &sh) {
+FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedShape &sh, const PlanTrace *replay,
+                     PlanTrace *record) {
   FusedPlan plan;
   plan.shape = sh;
   plan.n = n;
@@ -313,6 +314,8 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     uint64_t Q = 0;
     for (int p = 0; p < c; p++) Q |= (uint64_t)1 << logi[p];
     std::vector<int> included, deferred;
+    const size_t pass_index = plan.passes.size();
+    const bool replaying = replay && pass_index < replay->tiles.size() && pass_index < replay->rsets.size();
     auto scan_tile = [&](uint64_t &q, bool grow) {
       uint64_t blocked = 0;
       included.clear();
@@ -350,57 +353,66 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
       for (int x = 0; popc(q) < K && x < n; x++) q |= (uint64_t)1 << x;
     };
     const uint64_t forced = Q;
-    scan_tile(Q, true);
-    if (popc(Q) < K && !deferred.empty()) {
-      for (int idx : deferred) {
-        if (popc(Q) >= K) break;
-        if (G[idx].pair) Q |= (uint64_t)1 << G[idx].g.target;
-      }
-      fill(Q);
+    if (replaying) {
+      Q = replay->tiles[pass_index];
       scan_tile(Q, false);
-    }
-    fill(Q);
-    if (sh.search) {
-      // Tile search: grow the tile one qubit at a time, each time by the qubit
-      // whose addition lets the most pending gates run in this pass (ties: the
-      // qubit a pending gate needs first); keep it if it beats the in-order
-      // choice above.
-      auto runnable = [&](uint64_t q) {
-        uint64_t blocked = 0;
-        int count = 0;
-        for (size_t k = 0; k < pending.size() && blocked != all; k++) {
-          const PGate &x = G[pending[k]];
-          if ((x.touch & blocked) || (x.pair && !(q & ((uint64_t)1 << x.g.target)))) {
-            blocked |= x.touch;
-            continue;
-          }
-          count++;
+    } else {
+      scan_tile(Q, true);
+      if (popc(Q) < K && !deferred.empty()) {
+        for (int idx : deferred) {
+          if (popc(Q) >= K) break;
+          if (G[idx].pair) Q |= (uint64_t)1 << G[idx].g.target;
         }
-        return count;
-      };
-      int first_need[64];
-      for (int x = 0; x < 64; x++) first_need[x] = INT32_MAX;
-      for (size_t k = 0; k < pending.size(); k++) {
-        const PGate &x = G[pending[k]];
-        if (x.pair && first_need[x.g.target] == INT32_MAX) first_need[x.g.target] = (int)k;
-      }
-      uint64_t S = forced;
-      while (popc(S) < K) {
-        int best = -1, best_count = -1;
-        for (int x = 0; x < n; x++) {
-          if (S & ((uint64_t)1 << x)) continue;
-          const int cnt = runnable(S | ((uint64_t)1 << x));
-          if (cnt > best_count || (cnt == best_count && first_need[x] < first_need[best])) {
-            best = x;
-            best_count = cnt;
-          }
-        }
-        S |= (uint64_t)1 << best;
-      }
-      if (runnable(S) > (int)included.size()) {
-        Q = S;
+        fill(Q);
         scan_tile(Q, false);
       }
+      fill(Q);
+      if (sh.search) {
+        // Tile search: grow the tile one qubit at a time, each time by the qubit
+        // whose addition lets the most pending gates run in this pass (ties: the
+        // qubit a pending gate needs first); keep it if it beats the in-order
+        // choice above.
+        auto runnable = [&](uint64_t q) {
+          uint64_t blocked = 0;
+          int count = 0;
+          for (size_t k = 0; k < pending.size() && blocked != all; k++) {
+            const PGate &x = G[pending[k]];
+            if ((x.touch & blocked) || (x.pair && !(q & ((uint64_t)1 << x.g.target)))) {
+              blocked |= x.touch;
+              continue;
+            }
+            count++;
+          }
+          return count;
+        };
+        int first_need[64];
+        for (int x = 0; x < 64; x++) first_need[x] = INT32_MAX;
+        for (size_t k = 0; k < pending.size(); k++) {
+          const PGate &x = G[pending[k]];
+          if (x.pair && first_need[x.g.target] == INT32_MAX) first_need[x.g.target] = (int)k;
+        }
+        uint64_t S = forced;
+        while (popc(S) < K) {
+          int best = -1, best_count = -1;
+          for (int x = 0; x < n; x++) {
+            if (S & ((uint64_t)1 << x)) continue;
+            const int cnt = runnable(S | ((uint64_t)1 << x));
+            if (cnt > best_count || (cnt == best_count && first_need[x] < first_need[best])) {
+              best = x;
+              best_count = cnt;
+            }
+          }
+          S |= (uint64_t)1 << best;
+        }
+        if (runnable(S) > (int)included.size()) {
+          Q = S;
+          scan_tile(Q, false);
+        }
+      }
+    }
+    if (record) {
+      record->tiles.push_back(Q);
+      record->rsets.emplace_back();
     }
 
     int phys[64];
@@ -458,6 +470,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
       }
       uint32_t rset = 0;
       std::vector<int> here, next;
+      const size_t phase_index = rsets.size();
       auto scan_phase = [&](uint32_t &rs, bool grow) {
         uint64_t pblocked = 0;
         here.clear();
@@ -484,77 +497,83 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           here.push_back(idx);
         }
       };
-      scan_phase(rset, true);
-      if (popc(rset) < R && !next.empty()) {
-        // Spare register bits go to the targets of the deferred gates (in
-        // order), and the phase is rescanned: fewer phases, fewer transposes.
-        for (int idx : next) {
-          if (popc(rset) >= R) break;
-          if (G[idx].pair) rset |= 1u << local_of[G[idx].g.target];
-        }
+      if (replaying && phase_index < replay->rsets[pass_index].size()) {
+        rset = replay->rsets[pass_index][phase_index];
         scan_phase(rset, false);
-      }
-      if (sh.phase_search && !next.empty()) {
-        // Phase search: grow a register set position by position for the most
-        // gates runnable in this phase (same blocking rule; ties: the position
-        // a pending gate needs first), optionally improve it by single
-        // exchanges, and keep it if it runs more gates than the in-order set.
-        const int penalty = rsets.empty() ? sh.low_penalty : 0;
-        auto score = [&](uint32_t rs) {
-          uint64_t pb = 0;
-          int cnt = (rs & low_local) ? -penalty : 0;
-          for (int idx : remaining) {
-            const PGate &x = G[idx];
-            if ((x.touch & pb) || (x.pair && !(rs & (1u << local_of[x.g.target])))) {
-              pb |= x.touch;
-              continue;
-            }
-            cnt++;
+      } else {
+        scan_phase(rset, true);
+        if (popc(rset) < R && !next.empty()) {
+          // Spare register bits go to the targets of the deferred gates (in
+          // order), and the phase is rescanned: fewer phases, fewer transposes.
+          for (int idx : next) {
+            if (popc(rset) >= R) break;
+            if (G[idx].pair) rset |= 1u << local_of[G[idx].g.target];
           }
-          return cnt;
-        };
-        int first_need[32];
-        for (int i = 0; i < 32; i++) first_need[i] = INT32_MAX;
-        for (size_t k = 0; k < remaining.size(); k++) {
-          const PGate &x = G[remaining[k]];
-          const int lp = local_of[x.g.target];
-          if (x.pair && first_need[lp] == INT32_MAX) first_need[lp] = (int)k;
+          scan_phase(rset, false);
         }
-        uint32_t S = 0;
-        while (popc(S) < R) {
-          int best = -1, best_score = INT32_MIN;
-          for (int q = 0; q < K; q++) {
-            if (S & (1u << q)) continue;
-            const int v = score(S | (1u << q));
-            if (v > best_score || (v == best_score && first_need[q] < first_need[best])) {
-              best = q;
-              best_score = v;
+        if (sh.phase_search && !next.empty()) {
+          // Phase search: grow a register set position by position for the most
+          // gates runnable in this phase (same blocking rule; ties: the position
+          // a pending gate needs first), optionally improve it by single
+          // exchanges, and keep it if it runs more gates than the in-order set.
+          const int penalty = rsets.empty() ? sh.low_penalty : 0;
+          auto score = [&](uint32_t rs) {
+            uint64_t pb = 0;
+            int cnt = (rs & low_local) ? -penalty : 0;
+            for (int idx : remaining) {
+              const PGate &x = G[idx];
+              if ((x.touch & pb) || (x.pair && !(rs & (1u << local_of[x.g.target])))) {
+                pb |= x.touch;
+                continue;
+              }
+              cnt++;
             }
+            return cnt;
+          };
+          int first_need[32];
+          for (int i = 0; i < 32; i++) first_need[i] = INT32_MAX;
+          for (size_t k = 0; k < remaining.size(); k++) {
+            const PGate &x = G[remaining[k]];
+            const int lp = local_of[x.g.target];
+            if (x.pair && first_need[lp] == INT32_MAX) first_need[lp] = (int)k;
           }
-          S |= 1u << best;
-        }
-        int cur_score = score(S);
-        for (bool improved = sh.phase_search >= 2; improved;) {
-          improved = false;
-          for (int a = 0; a < K && !improved; a++) {
-            if (!(S & (1u << a))) continue;
-            for (int b = 0; b < K && !improved; b++) {
-              if (S & (1u << b)) continue;
-              const uint32_t T = (S & ~(1u << a)) | (1u << b);
-              const int v = score(T);
-              if (v > cur_score) {
-                S = T;
-                cur_score = v;
-                improved = true;
+          uint32_t S = 0;
+          while (popc(S) < R) {
+            int best = -1, best_score = INT32_MIN;
+            for (int q = 0; q < K; q++) {
+              if (S & (1u << q)) continue;
+              const int v = score(S | (1u << q));
+              if (v > best_score || (v == best_score && first_need[q] < first_need[best])) {
+                best = q;
+                best_score = v;
+              }
+            }
+            S |= 1u << best;
+          }
+          int cur_score = score(S);
+          for (bool improved = sh.phase_search >= 2; improved;) {
+            improved = false;
+            for (int a = 0; a < K && !improved; a++) {
+              if (!(S & (1u << a))) continue;
+              for (int b = 0; b < K && !improved; b++) {
+                if (S & (1u << b)) continue;
+                const uint32_t T = (S & ~(1u << a)) | (1u << b);
+                const int v = score(T);
+                if (v > cur_score) {
+                  S = T;
+                  cur_score = v;
+                  improved = true;
+                }
               }
             }
           }
-        }
-        if (cur_score > (int)here.size() - ((rset & low_local) ? penalty : 0)) {
-          rset = S;
-          scan_phase(rset, false);
+          if (cur_score > (int)here.size() - ((rset & low_local) ? penalty : 0)) {
+            rset = S;
+            scan_phase(rset, false);
+          }
         }
       }
+      if (record) record->rsets[pass_index].push_back(rset);
       // Fill the register set from the top local positions (keeps the
       // contiguous low bits on lanes, so loads and stores stay coalesced).
       for (int p = K - 1; popc(rset) < R && p >= 0; p--)

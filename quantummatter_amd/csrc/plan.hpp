@@ -225,8 +225,21 @@ struct FusedPlan {
   std::vector<TileOp> flat_ops();
 };
 
+// The searched decisions of a plan: each pass's tile (qubit mask) and each
+// phase's register set (tile-local mask, before unused slots are filled).
+// The planner's decisions depend on the gates' kinds, targets and controls
+// only, never on their coefficients, so a trace recorded for one circuit
+// replays exactly for every circuit of the same structure (new angles).
+struct PlanTrace {
+  std::vector<uint64_t> tiles;
+  std::vector<std::vector<uint32_t>> rsets;
+};
+
 // Plans prepared gates (identity removed) for an n-qubit state, n >= shape.K.
-FusedPlan plan_fused(const std::vector<DevGate> &gates, int n, const FusedShape &shape);
+// With `replay`, the tile and phase searches are skipped and the recorded
+// decisions are taken; with `record`, the decisions are written there.
+FusedPlan plan_fused(const std::vector<DevGate> &gates, int n, const FusedShape &shape,
+                     const PlanTrace *replay = nullptr, PlanTrace *record = nullptr);
 
 // ---------------------------------------------------------------------------
 // Batched programs: one program run on many same-width states, each state by

@@ -94,13 +94,17 @@ struct Ctx {
   double last_dense_flops = 0;  // matrix-core flops of the last dense execution
   // Tile shape per precision (qm_ctx_set_fused_shape).  The compiled schedule
   // uses it as given; the interpreting kernel uses it when it is one of its
-  // instantiations and its own default otherwise.  Defaults: 16384-amplitude
-  // tiles, 512 threads x 32 registers (one workgroup per CU: the exchange
-  // buffer takes 128 KiB of LDS in complex128), the low 3 (complex128, or 4
-  // when that plan has fewer passes) or 5 (complex64) index bits remapped
-  // between passes: the compiled schedule's best on the 30-qubit layer
-  // workload (14 and 19 passes for 40 layers).
-  FusedShape shape[2] = {alt_low_bits(compiled_shape(14, 9, 3), 4), compiled_shape(14, 9, 5)};
+  // instantiations and its own default otherwise.  Defaults: complex128
+  // 8192-amplitude tiles, 256 threads x 32 registers (two workgroups per CU,
+  // each with a 64 KiB LDS exchange buffer, so one tile's loads and stores
+  // overlap the other's arithmetic), the low 3 index bits (or 4 when that plan
+  // models faster) remapped between passes; complex64 16384-amplitude tiles,
+  // 512 threads x 32 registers (two workgroups per CU), the low 5 bits.  Over
+  // six layer circuits of 26-30 qubits the complex128 shape runs 1-23 %
+  // faster than 16384-amplitude tiles with one workgroup per CU (18 passes of
+  // the 30-qubit 40-layer circuit against 14, each shorter); in complex64 the
+  // 16384-amplitude shape is the faster (19 passes).
+  FusedShape shape[2] = {alt_low_bits(compiled_shape(13, 8, 3), 4), compiled_shape(14, 9, 5)};
   // Shapes chosen by qm_autotune, [fused 0 / compiled 1][precision][qubits];
   // K == 0: not tuned.  They take precedence over `shape` for their width.
   FusedShape tuned[2][2][35] = {};

@@ -362,9 +362,10 @@ def test_dense_blocks_layer_circuit_16q():
 def test_compiled_schedule_plans_with_both_low_bit_counts():
     """The complex128 compiled schedule plans with 3 and with 4 low bits kept
     in the tiles (full and 9-phase budgets) and runs the plan of lowest
-    modelled time; on this circuit that is the 4-bit plan (12 passes against
-    13 with 3 bits; qm_jit_compile_check plans with 4), and the state matches
-    the per-gate schedule."""
+    modelled time; on this circuit that is the 3-bit plan at the default
+    8192-amplitude tiles (14 passes against 16 with 4 bits; qm_jit_compile_check
+    plans with 4), and the state matches the per-gate schedule, also when the
+    plan is replayed from the cached decisions (second call, new angles)."""
     import ctypes as C
     from quantummatter_amd._lib import lib, ptr
     from quantummatter_amd.backend import (SCHED_COMPILED, SCHED_PER_GATE, HipBackendOptions,
@@ -372,12 +373,21 @@ def test_compiled_schedule_plans_with_both_low_bit_counts():
     n = 26
     ops = workloads.random_layer_circuit(n, 40, seed=2).to_ops()
     four = C.c_int32()
-    assert lib().qm_jit_compile_check(n, 0, ptr(ops), len(ops), 14, 9, C.byref(four)) == 0
     with HipStateVectorBackend(HipBackendOptions(EnableProfiling=True)) as be:
+        K, L = be.GetFusedShape(C128)
+        assert lib().qm_jit_compile_check(n, 0, ptr(ops), len(ops), K, L, C.byref(four)) == 0
         with be.CreateSession(n, C128) as a, be.CreateSession(n, C128) as b:
             a.EvolveOps(ops, SCHED_PER_GATE)
             p = b.EvolveOps(ops, SCHED_COMPILED, profile=True)
             assert p.Passes <= four.value
+            assert abs(1 - abs(b.Inner(a))) <= 1e-12
+            ops2 = ops.copy()
+            ops2["angle"] = ops2["angle"] * 0.7 + 0.3
+            a.Reset()
+            b.Reset()
+            a.EvolveOps(ops2, SCHED_PER_GATE)
+            p2 = b.EvolveOps(ops2, SCHED_COMPILED, profile=True)
+            assert p2.Passes == p.Passes
             assert abs(1 - abs(b.Inner(a))) <= 1e-12
 
 
