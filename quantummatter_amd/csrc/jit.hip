@@ -222,8 +222,8 @@ std::string hex(uint64_t v) {
 }
 
 // Source of one pass kernel, and the coefficient layout it reads.
-// Code-generation variant (tuning knob QM_JIT_VARIANT; default 3473 = bits 0,
-// 4, 7, 8, 10 and 11 in complex128, 3985 = those and bit 9 in complex64):
+// Code-generation variant (tuning knob QM_JIT_VARIANT; default 11665 = bits 0,
+// 4, 7, 8, 10, 11 and 13 in complex128, 12177 = those and bit 9 in complex64):
 // bit 0 exchanges through an LDS buffer of reals (half the LDS,
 // real then imaginary parts, so more workgroups fit a CU), bit 2 forms the
 // lane part of each LDS address at its transpose instead of once per kernel,
@@ -261,13 +261,17 @@ std::string hex(uint64_t v) {
 // under a branch, with each exchange's layout chosen for the folded writer
 // addresses and a per-thread factor for flipped diagonals (the moves of the
 // conditional swaps go, 7 % fewer VALU instructions, but 2 % slower in
-// complex128 and 5 % in complex64: more spills and waits).
+// complex128 and 5 % in complex64: more spills and waits), bit 13 (with bit 4,
+// without bit 5) holds factored diagonals on register bits back while the ops
+// that follow do not target their bits and applies them together, one product
+// per register (27 % fewer of their complex products on the 30-qubit layer
+// circuit; 1-3 % faster in complex128, 4 % in complex64).
 int jit_variant(int precision) {
   static const int v = [] {
     const char *e = std::getenv("QM_JIT_VARIANT");
     return e ? std::atoi(e) : -1;
   }();
-  return v >= 0 ? v : precision == QM_C64 ? 3985 : 3473;
+  return v >= 0 ? v : precision == QM_C64 ? 12177 : 11665;
 }
 
 int variant_of(const FusedShape &sh, int precision) {
@@ -427,6 +431,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const bool banked = (variant & 1024) != 0 && split;
   const bool scalar_coef = (variant & 2048) != 0 && tile_coef;
   const bool fold_frames = (variant & 4096) != 0 && frames && banked;
+  const bool merge_diag = (variant & 8192) != 0 && !frames && factored;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
   const bool c128 = precision == QM_C128;
@@ -779,6 +784,42 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   // half by the inverse ratio and the thread's factor by the ratio, and a
   // factored Ry in form b on a flipped position negates the thread's factor,
   // instead of multiplying both halves.
+  // Merged register diagonals (merge_diag): a factored diagonal on register
+  // bit b (multiply the registers with bit b set by q) commutes with every op
+  // that does not target b as a pair, so it is held back and combined with
+  // the other held diagonals; when an op targets a held bit, or the phase
+  // ends, each register is multiplied once by the product of the held ratios
+  // of its set bits (2^k - 1 distinct products for k held bits, formed on the
+  // host): (2^k - 1) N / 2^k complex products instead of k N / 2.
+  std::vector<std::pair<int, std::pair<double, double>>> held;  // (register bit, ratio)
+  auto flush_held = [&]() {
+    if (held.empty()) return;
+    const int k = (int)held.size();
+    std::vector<size_t> slot((size_t)1 << k, 0);
+    for (int m = 1; m < (1 << k); m++) {
+      double pr = 1, pi = 0;
+      for (int i = 0; i < k; i++)
+        if ((m >> i) & 1) {
+          const double ar = held[(size_t)i].second.first, ai = held[(size_t)i].second.second;
+          const double nr = pr * ar - pi * ai, ni = pr * ai + pi * ar;
+          pr = nr;
+          pi = ni;
+        }
+      const double v[2] = {pr, pi};
+      slot[(size_t)m] = cfix(v, 2);
+    }
+    o << "    {";
+    for (int r = 0; r < N; r++) {
+      int m = 0;
+      for (int i = 0; i < k; i++)
+        if ((r >> held[(size_t)i].first) & 1) m |= 1 << i;
+      if (!m) continue;
+      o << " qj_mul(vr[" << r << "], vi[" << r << "], " << cf_str(slot[(size_t)m]) << ", "
+        << cf_str(slot[(size_t)m] + 1) << ");";
+    }
+    o << " }\n";
+    held.clear();
+  };
   for (int ph = 0; ph < d.nphases; ph++) {
     if (cur != ph) move_to(cur, ph);
     cur = ph;
@@ -788,6 +829,34 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     if (gl_ok) o << "    F " << glr << " = (F)1, " << gli << " = (F)0;\n";
     for (int i = 0; i < d.phase_op_count[ph]; i++) {
       const TileOp &op = ops[d.phase_op_begin[ph] + i];
+      if (merge_diag) {
+        if (factor_diag(op)) {
+          // diag(A, D) = A diag(1, D/A): A joins the pass's factor, D/A is held.
+          const double ar = op.c[0], ai = op.c[1], dr = op.c[6], di = op.c[7];
+          const double den = ar * ar + ai * ai;
+          const double qr = (dr * ar + di * ai) / den, qi = (di * ar - dr * ai) / den;
+          const double nr = gre * ar - gim * ai, ni = gre * ai + gim * ar;
+          gre = nr;
+          gim = ni;
+          const int b = __builtin_ctz(op.dreg);
+          bool merged = false;
+          for (auto &h : held)
+            if (h.first == b) {
+              const double hr = h.second.first * qr - h.second.second * qi;
+              const double hi = h.second.first * qi + h.second.second * qr;
+              h.second = {hr, hi};
+              merged = true;
+            }
+          if (!merged) held.push_back({b, {qr, qi}});
+          continue;
+        }
+        if (op.rbit >= 0)
+          for (const auto &h : held)
+            if (h.first == op.rbit) {
+              flush_held();
+              break;
+            }
+      }
       // An X on register bit b controlled by one register bit c whose
       // position is flipped under condition f: X_b^(r_c xor f) is the
       // register permutation X_b^(r_c) followed by the flip X_b^f, so the
@@ -955,6 +1024,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       }
       o << " }\n";
     }
+    flush_held();
     const int dcnt = d.phase_diag_count[ph];
     const bool last = ph == d.nphases - 1;
     std::string g0 = "1", g1 = "0";

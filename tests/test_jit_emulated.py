@@ -188,11 +188,12 @@ def _variant_case(variant, kind, n, seed, tmp_path):
                                                  (913, "layer", 16, 13), (1425, "all", 15, 14),
                                                  (1425, "layer", 16, 15), (1937, "all", 15, 16),
                                                  (3473, "all", 15, 17), (3985, "layer", 16, 18),
-                                                 (7601, "all", 15, 19), (7601, "layer", 16, 20)])
+                                                 (7601, "all", 15, 19), (7601, "layer", 16, 20),
+                                                 (11665, "all", 15, 21), (12177, "layer", 16, 22)])
 def test_generated_kernel_variants_match_oracle(tmp_path, variant, kind, n, seed):
     """Optional code variants: factored forms (bit 4), exchange flags (bit 5),
     per-tile coefficient loads (bit 7), per-exchange lane addresses (bit 8),
     folded LDS offsets (bit 9), scalar coefficient loads (bit 11) and flips
     carried through register CNOTs and folded into exchange and store
-    addresses (bit 12)."""
+    addresses (bit 12) and merged register diagonals (bit 13)."""
     assert _variant_case(variant, kind, n, seed, tmp_path) <= 1e-12
