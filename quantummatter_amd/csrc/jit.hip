@@ -4,8 +4,12 @@
 // coefficients read from memory.  This removes the per-op dispatch of the
 // interpreting kernel (fused.hip): no runtime switch over register bits, so
 // no register shuffling between ops, and register controls select at compile
-// time.  Kernels are cached by their source text (the pass structure without
-// coefficients), so re-running a circuit structure with new angles reuses them.
+// time.  Kernels are cached by their source text: the pass structure and, for
+// each uncontrolled rotation taken in factored form, which of its two forms
+// its angle selects (the larger of |cos| and |sin| of half the angle is taken
+// out), not the coefficients; so re-running a circuit structure with new
+// angles reuses the kernels of every pass whose rotations keep their forms,
+// and compiles the others.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 #include <dlfcn.h>
