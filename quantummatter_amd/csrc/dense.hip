@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "dense.hpp"
@@ -68,7 +69,11 @@ __device__ __forceinline__ constexpr int partner_reg(int mt, int i) {
   return -1;
 }
 
-template <typename T, int K>
+// TPI column tiles per loop iteration, their loads issued together so more
+// of the state is in flight per wave.  The best count depends on the width
+// and the precision (the registers the matrix operand and the loads take set
+// the occupancy); dense_tiles holds the measured choice.
+template <typename T, int K, int TPI>
 __global__ __launch_bounds__(256) void k_dense(T *__restrict__ s, const typename DenseMfma<T>::R *__restrict__ A,
                                                const DenseArgs args) {
   using Mf = DenseMfma<T>;
@@ -83,13 +88,16 @@ __global__ __launch_bounds__(256) void k_dense(T *__restrict__ s, const typename
   const int cl = lane & 15, grp = lane >> 4;
   const uint64_t tiles = (args.ncols + 15) / 16;
   const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-  for (uint64_t ct = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave; ct < tiles; ct += waves) {
+  // Column base of a lane's column in tile ct (state bits of the non-block
+  // qubits), and whether the column exists.
+  auto column = [&](uint64_t ct, uint64_t &base) {
     const uint64_t col = ct * 16 + cl;
-    const bool valid = col < args.ncols;
-    uint64_t base = 0;
+    base = 0;
     for (int r = 0; r < args.nruns; r++)
       base |= ((col >> args.run_src[r]) & ((1ull << args.run_len[r]) - 1)) << args.run_dst[r];
-    R b[KS];
+    return col < args.ncols;
+  };
+  auto load_b = [&](R (&b)[KS], bool valid, uint64_t base) {
     if constexpr (K >= 2) {
       // B rows 4 kk + grp (kk < KS / 2) are the real parts of local index
       // 4 kk + grp, rows 4 kk + grp + D their imaginary parts.
@@ -108,6 +116,9 @@ __global__ __launch_bounds__(256) void k_dense(T *__restrict__ s, const typename
         b[kk] = valid ? sr[2 * (base | args.loc_off[local]) + part] : (R)0;
       }
     }
+  };
+  // Y = U X for one tile: the products, then the stores of a valid column.
+  auto product = [&](const R (&b)[KS], bool valid, uint64_t base) {
     Acc acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; mt++) acc[mt] = Acc{0, 0, 0, 0};
@@ -120,7 +131,7 @@ __global__ __launch_bounds__(256) void k_dense(T *__restrict__ s, const typename
         acc[mt] = Mf::mfma(a, b[kk], acc[mt]);
       }
     }
-    if (!valid) continue;
+    if (!valid) return;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++) {
 #pragma unroll
@@ -142,16 +153,54 @@ __global__ __launch_bounds__(256) void k_dense(T *__restrict__ s, const typename
         }
       }
     }
+  };
+  const uint64_t first = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  for (uint64_t ct0 = first; ct0 < tiles; ct0 += waves * TPI) {
+    R b[TPI][KS];
+    bool valid[TPI];
+    uint64_t base[TPI];
+#pragma unroll
+    for (int u = 0; u < TPI; u++) {
+      valid[u] = false;
+      base[u] = 0;
+      if (ct0 + u * waves < tiles) valid[u] = column(ct0 + u * waves, base[u]);
+      load_b(b[u], valid[u], base[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < TPI; u++)
+      if (ct0 + u * waves < tiles) product(b[u], valid[u], base[u]);
   }
+}
+
+// Column tiles per iteration by precision and width, measured on the
+// 30-qubit layer circuit with blocks of k = 2..5 (tuning knob QM_DENSE_TPI:
+// 1, 2 or 4 for every width).
+int dense_tiles(int precision, int k) {
+  static const int knob = [] {
+    const char *e = std::getenv("QM_DENSE_TPI");
+    return e ? std::atoi(e) : 0;
+  }();
+  if (knob == 1 || knob == 2 || knob == 4) return knob;
+  if (precision == QM_C128) return k <= 2 ? 4 : 2;
+  return k <= 3 ? 4 : k == 4 ? 2 : 1;
+}
+
+template <typename T, int K, int TPI>
+void launch_dense_v(State *st, const DenseArgs &a, const void *d_matrix, hipStream_t stream) {
+  const uint64_t tiles = (a.ncols + 15) / 16;
+  const uint64_t blocks = std::min<uint64_t>((tiles + 3) / 4, (uint64_t)st->ctx->compute_units * 8);
+  k_dense<T, K, TPI><<<(unsigned)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(
+      (T *)st->data, (const typename DenseMfma<T>::R *)d_matrix, a);
+  QM_HIP(hipGetLastError());
 }
 
 template <typename T, int K>
 void launch_dense_t(State *st, const DenseArgs &a, const void *d_matrix, hipStream_t stream) {
-  const uint64_t tiles = (a.ncols + 15) / 16;
-  const uint64_t blocks = std::min<uint64_t>((tiles + 3) / 4, (uint64_t)st->ctx->compute_units * 8);
-  k_dense<T, K><<<(unsigned)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(
-      (T *)st->data, (const typename DenseMfma<T>::R *)d_matrix, a);
-  QM_HIP(hipGetLastError());
+  switch (dense_tiles(st->precision, K)) {
+    case 4: launch_dense_v<T, K, 4>(st, a, d_matrix, stream); break;
+    case 2: launch_dense_v<T, K, 2>(st, a, d_matrix, stream); break;
+    default: launch_dense_v<T, K, 1>(st, a, d_matrix, stream);
+  }
 }
 
 template <typename T>
