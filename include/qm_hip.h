@@ -285,7 +285,10 @@ int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n
  * bits; the state is back in natural order at the end), bit 2 the tile search
  * (each pass's tile grown for the most runnable gates), bit 3 the replay check
  * (the plan is built from decisions recorded on the same gates with other
- * coefficients, must equal a fresh plan, and is the one applied). */
+ * coefficients, must equal a fresh plan, and is the one applied), bit 4 the
+ * compiled schedule's searched planning (perturbed restarts and iterated
+ * greedy moves over the tile choices, a small budget; the plan replayed from
+ * its recorded trace must equal the searched plan, and is the one applied). */
 int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
                        int32_t lane_bits, int32_t contiguous, int32_t flags, double *amplitudes);
 

@@ -279,6 +279,48 @@ FusedPlan plan_cached(const std::vector<DevGate> &gates, int n, int precision, c
   plan_cache_put(key, std::move(rec));
   return plan;
 }
+// Searched planning of the compiled schedule (plan_fused_searched), cached by
+// structure.  Base shapes: the shape's low-bit count and its alternative, each
+// with the full phase budget and with at most 9 phases per pass.  Cost: the
+// pass-time model fitted on nine plans of the 30-qubit c128 layer circuit at
+// the default 8192-amplitude tiles with two workgroups per CU (0.077 ms per
+// gate, the same in every plan, + 1.8 ms per pass + 0.26 ms per LDS exchange
+// over a 16 GiB state; within 2.5 %).  Search budget knobs: QM_PLAN_RESTARTS
+// (perturbed restarts, default 96; 0 plans the base shapes only) and
+// QM_PLAN_ROUNDS (rounds of 16 iterated greedy moves, default 8).
+double compiled_plan_cost(const FusedPlan &p) {
+  double t = 0;
+  for (const FusedPass &x : p.passes) t += 1.8 + 0.26 * x.transposes;
+  return t;
+}
+FusedPlan plan_compiled(const std::vector<DevGate> &gates, int n, const FusedShape &shape, const std::string &key) {
+  static const PlanSearch opt = [] {
+    PlanSearch o;
+    o.restarts = 96;
+    o.rounds = 8;
+    if (const char *e = std::getenv("QM_PLAN_RESTARTS")) o.restarts = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("QM_PLAN_ROUNDS")) o.rounds = std::max(0, std::atoi(e));
+    if (o.restarts == 0) o.rounds = 0;
+    return o;
+  }();
+  std::vector<int> lows{shape.contiguous}, caps{shape.max_phases};
+  if (shape.alt_contiguous > 0 && shape.alt_contiguous < shape.K && shape.alt_contiguous != shape.contiguous)
+    lows.push_back(shape.alt_contiguous);
+  if (shape.max_phases != 9) caps.push_back(9);
+  std::vector<FusedShape> bases;
+  for (int low : lows)
+    for (int cap : caps) {
+      FusedShape b = shape;
+      b.contiguous = low;
+      b.max_phases = cap;
+      bases.push_back(b);
+    }
+  if (bases.empty()) bases.push_back(shape);
+  CachedPlan rec{shape, {}};
+  FusedPlan plan = plan_fused_searched(gates, n, bases, opt, compiled_plan_cost, &rec.shape, &rec.trace);
+  plan_cache_put(key, std::move(rec));
+  return plan;
+}
 }  // namespace
 
 // Runs prepared gates on a resident state with the chosen schedule.
@@ -338,44 +380,25 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
     const std::string key = plan_key(gates, s->qubits, s->precision, shape, QM_SCHED_COMPILED);
     CachedPlan hit;
     const bool known = plan_cache_get(key, &hit);
-    CachedPlan rec{shape, {}};
-    FusedPlan plan = known ? plan_fused(gates, s->qubits, hit.shape, &hit.trace)
-                           : plan_fused(gates, s->qubits, shape, nullptr, &rec.trace);
-    if (!known && shape.alt_contiguous > 0 && shape.alt_contiguous < shape.K &&
-        shape.alt_contiguous != shape.contiguous) {
-      // Candidate plans: both low-bit counts, each with the full phase budget
-      // and with at most 9 phases per pass; the cheapest under the pass-time
-      // model runs.  Model, fitted on four plans of the 30-qubit c128 layer
-      // circuit with searched phases (14 to 19 passes, 72 to 101 exchanges;
-      // 54 ms + 5.1 ms per pass + 0.87 ms per LDS exchange over a 16 GiB
-      // state, the constant being the gates' arithmetic, the same in every
-      // plan; within 1 % of the four measurements).
-      auto cost = [](const FusedPlan &p) {
-        double t = 0;
-        for (const FusedPass &x : p.passes) t += 5.1 + 0.87 * x.transposes;
-        return t;
-      };
-      double best = cost(plan);
-      for (int low : {shape.contiguous, shape.alt_contiguous})
-        for (int phases : {0, 9}) {
-          if (low == shape.contiguous && phases == shape.max_phases) continue;  // the first plan
-          FusedShape alt = shape;
-          alt.contiguous = low;
-          alt.max_phases = phases;
-          PlanTrace trace;
-          FusedPlan other = plan_fused(gates, s->qubits, alt, nullptr, &trace);
-          const double c = cost(other);
-          if (c < best) {
-            best = c;
-            plan = std::move(other);
-            rec = CachedPlan{alt, std::move(trace)};
-          }
-        }
+    FusedPlan plan;
+    bool have_plan = false;
+    if (known) {
+      plan = plan_fused(gates, s->qubits, hit.shape, &hit.trace);
+      have_plan = true;
+    } else if (!automatic) {
+      plan = plan_compiled(gates, s->qubits, shape, key);
+      have_plan = true;
+    } else if (auto_compile()) {
+      // The search and the compilation run in the background; this call runs
+      // interpreted, and a later call of the structure replays the searched plan.
+      const int n = s->qubits;
+      const FusedShape base = shape;
+      jit_plan_async(key, [gates, n, base, key] { return plan_compiled(gates, n, base, key); }, s->precision,
+                     c->device);
     }
-    if (!known) plan_cache_put(key, std::move(rec));
     JitProgram jp;
     bool ready = false;
-    if (jit_supported(plan)) {
+    if (have_plan && jit_supported(plan)) {
       if (!automatic) {
         jp = jit_prepare(plan, s->precision, c->device);
         ready = true;
@@ -1279,12 +1302,35 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
     FusedShape shape{tile_bits, lane_bits, tile_bits - lane_bits, contiguous};
     shape.remap = flags & 3;
     shape.search = (flags >> 2) & 1;
-    if (flags < 0 || flags > 15 || shape.remap == 3) fail(QM_ERR_RANGE, "unsupported planning flags");
+    if (flags < 0 || flags > 31 || shape.remap == 3) fail(QM_ERR_RANGE, "unsupported planning flags");
     if (shape.R < 1 || shape.R > 5 || shape.L < 1 || shape.L > 10 || shape.K > kMaxK ||
         contiguous < 0 || contiguous >= shape.K)
       fail(QM_ERR_RANGE, "unsupported tile shape");
     if (qubits < shape.K) {
       for (const DevGate &x : g) apply_gate_host(x, amps, qubits);
+      return;
+    }
+    if (flags & 16) {
+      // Searched plan (plan_fused_searched, a small budget): the plan replayed
+      // from its trace must equal the searched plan, and is the one applied.
+      PlanSearch opt;
+      opt.restarts = 12;
+      opt.rounds = 2;
+      opt.moves = 6;
+      FusedShape capped = shape;
+      capped.max_phases = 9;
+      FusedShape chosen;
+      PlanTrace trace;
+      FusedPlan searched = plan_fused_searched(g, qubits, {shape, capped}, opt, compiled_plan_cost, &chosen, &trace);
+      FusedPlan replayed = plan_fused(g, qubits, chosen, &trace);
+      bool same = replayed.passes.size() == searched.passes.size();
+      for (size_t i = 0; same && i < searched.passes.size(); i++) {
+        const FusedPass &a = replayed.passes[i], &b = searched.passes[i];
+        same = std::memcmp(&a.dev, &b.dev, sizeof(DevPass)) == 0 && a.ops.size() == b.ops.size() &&
+               (a.ops.empty() || std::memcmp(a.ops.data(), b.ops.data(), a.ops.size() * sizeof(TileOp)) == 0);
+      }
+      if (!same) fail(QM_ERR_DEVICE, "a replayed searched plan differs from the searched plan");
+      apply_plan_host(replayed, amps, qubits);
       return;
     }
     if (flags & 8) {

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <sstream>
@@ -1301,6 +1302,37 @@ std::mutex &async_serial() {
 }
 }  // namespace
 
+// Background job: `make` produces the plan (possibly searching for it) and the
+// plan's missing kernels are then compiled; jobs of one tag run once at a time,
+// and at most four jobs are in flight.
+static void jit_job(const std::string &tag, std::function<FusedPlan()> make, int precision, int device) {
+  AsyncJit &a = async_jit();
+  {
+    std::lock_guard<std::mutex> lk(a.mu);
+    if (a.queued.count(tag) || a.running >= 4) return;  // already queued, or the queue is full
+    a.queued[tag] = 1;
+    a.running++;
+  }
+  std::thread([make, precision, device, tag] {
+    try {
+      const FusedPlan plan = make();
+      if (!plan.passes.empty() && jit_supported(plan)) {
+        std::lock_guard<std::mutex> serial(async_serial());
+        (void)hipSetDevice(device);
+        (void)jit_prepare(plan, precision, device);
+      }
+    } catch (...) {
+      // A failed background plan or compile only means the next execution
+      // plans or compiles again (and reports the failure) or runs interpreted.
+    }
+    AsyncJit &a = async_jit();
+    std::lock_guard<std::mutex> lk(a.mu);
+    a.queued.erase(tag);
+    a.running--;
+    a.idle.notify_all();
+  }).detach();
+}
+
 void jit_prepare_async(const FusedPlan &plan, int precision, int device) {
   if (plan.passes.empty()) return;
   std::string tag;
@@ -1310,30 +1342,13 @@ void jit_prepare_async(const FusedPlan &plan, int precision, int device) {
     plan_keys(plan, precision, device, jp, keys);
     for (const std::string &k : keys) tag += std::to_string(std::hash<std::string>{}(k)) + ",";
   }
-  AsyncJit &a = async_jit();
-  {
-    std::lock_guard<std::mutex> lk(a.mu);
-    if (a.queued.count(tag) || a.running >= 4) return;  // already queued, or the queue is full
-    a.queued[tag] = 1;
-    a.running++;
-  }
-  std::thread([plan, precision, device, tag] {
-    {
-      std::lock_guard<std::mutex> serial(async_serial());
-      try {
-        (void)hipSetDevice(device);
-        (void)jit_prepare(plan, precision, device);
-      } catch (...) {
-        // A failed background compile only means the next execution compiles
-        // (and reports the failure) or runs interpreted.
-      }
-    }
-    AsyncJit &a = async_jit();
-    std::lock_guard<std::mutex> lk(a.mu);
-    a.queued.erase(tag);
-    a.running--;
-    a.idle.notify_all();
-  }).detach();
+  jit_job(tag, [plan] { return plan; }, precision, device);
+}
+
+void jit_plan_async(const std::string &structure, std::function<FusedPlan()> make, int precision,
+                    int device) {
+  jit_job("plan:" + std::to_string(device) + ":" + std::to_string(std::hash<std::string>{}(structure)),
+          std::move(make), precision, device);
 }
 
 void jit_drain() {

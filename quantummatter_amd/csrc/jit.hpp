@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <string>
 #include <vector>
 
 #include "plan.hpp"
@@ -27,6 +29,9 @@ bool jit_lookup(const FusedPlan &plan, int precision, int device, JitProgram *ou
 // Compiles the plan's missing kernels on a background thread (at most a few
 // plans at once; a plan already queued is not queued again).
 void jit_prepare_async(const FusedPlan &plan, int precision, int device);
+// Plans in the background (make() may search) and compiles the plan's missing
+// kernels; a job for the same structure key is not queued twice.
+void jit_plan_async(const std::string &structure, std::function<FusedPlan()> make, int precision, int device);
 // Waits until no background compilation is running.
 void jit_drain();
 void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,

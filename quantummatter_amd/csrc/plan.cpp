@@ -2,10 +2,13 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <exception>
 #include <stdexcept>
+#include <thread>
 
 #include "gate_math.hpp"
 
@@ -304,6 +307,15 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
   std::vector<PGate> G = expand(gates_in);
   std::vector<int> pending(G.size());
   for (size_t i = 0; i < G.size(); i++) pending[i] = (int)i;
+  // Perturbation stream of a restarted tile search (xorshift64 from the seed).
+  uint64_t rng = sh.search_seed * 0x9E3779B97F4A7C15ull + 1;
+  if (rng == 0) rng = 1;
+  auto perturb = [&rng] {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
 
   // A last pass with no gates restores the natural order if the final gate
   // pass could not.
@@ -396,7 +408,8 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           int best = -1, best_count = -1;
           for (int x = 0; x < n; x++) {
             if (S & ((uint64_t)1 << x)) continue;
-            const int cnt = runnable(S | ((uint64_t)1 << x));
+            int cnt = runnable(S | ((uint64_t)1 << x));
+            if (sh.search_seed) cnt += (int)(perturb() % (uint64_t)(kSearchNoise + 1));
             if (cnt > best_count || (cnt == best_count && first_need[x] < first_need[best])) {
               best = x;
               best_count = cnt;
@@ -833,6 +846,90 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     for (int x = 0; x < 64; x++) logi[x] = newlogi[x];
   }
   return plan;
+}
+
+FusedPlan plan_fused_searched(const std::vector<DevGate> &gates, int n, const std::vector<FusedShape> &bases,
+                              const PlanSearch &opt, double (*cost)(const FusedPlan &), FusedShape *chosen,
+                              PlanTrace *trace) {
+  if (bases.empty()) fail(QM_ERR_ARG, "no tile shape to plan with");
+  struct Try {
+    FusedShape shape;
+    PlanTrace prefix;
+    bool use_prefix = false;
+    FusedPlan plan;
+    PlanTrace rec;
+    double c = 0;
+    std::exception_ptr err;
+  };
+  int threads = opt.threads > 0 ? opt.threads : (int)std::thread::hardware_concurrency();
+  threads = std::max(1, std::min(threads, 16));
+  // Every try is planned independently (plan_fused keeps no shared state), so
+  // the result does not depend on which thread ran which try.
+  auto run = [&](std::vector<Try> &tries) {
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+      for (size_t i; (i = next.fetch_add(1)) < tries.size();) {
+        Try &t = tries[i];
+        try {
+          t.plan = plan_fused(gates, n, t.shape, t.use_prefix ? &t.prefix : nullptr, &t.rec);
+          t.c = cost(t.plan);
+        } catch (...) {
+          t.err = std::current_exception();
+        }
+      }
+    };
+    const int helpers = std::min<int>(threads, (int)tries.size()) - 1;
+    std::vector<std::thread> pool;
+    for (int i = 0; i < helpers; i++) pool.emplace_back(work);
+    work();
+    for (std::thread &th : pool) th.join();
+  };
+  Try best;
+  bool have = false;
+  // Lowest cost first, then fewer passes; ties keep the earlier try.
+  auto better = [&](const Try &t) {
+    if (t.err) return false;
+    if (!have) return true;
+    if (t.c != best.c) return t.c < best.c;
+    return t.plan.passes.size() < best.plan.passes.size();
+  };
+  uint64_t mix = opt.seed;
+  auto next_seed = [&mix] {
+    uint64_t z = (mix += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z ? z : 1;
+  };
+  // (1) the base shapes, deterministic; (2) perturbed restarts over them.
+  std::vector<Try> tries(bases.size() + (size_t)std::max(0, opt.restarts));
+  for (size_t i = 0; i < tries.size(); i++) {
+    tries[i].shape = bases[i % bases.size()];
+    tries[i].shape.search_seed = i < bases.size() ? 0 : next_seed();
+  }
+  run(tries);
+  for (size_t i = 0; i < bases.size(); i++)
+    if (tries[i].err) std::rethrow_exception(tries[i].err);  // a base shape must plan
+  for (Try &t : tries)
+    if (better(t)) best = std::move(t), have = true;
+  // (3) iterated greedy moves from the best plan.
+  for (int round = 0; round < opt.rounds && best.rec.tiles.size() > 1; round++) {
+    std::vector<Try> moves((size_t)std::max(0, opt.moves));
+    for (Try &m : moves) {
+      const size_t k = (size_t)(next_seed() % best.rec.tiles.size());
+      m.shape = best.shape;
+      m.shape.search_seed = next_seed();
+      m.prefix.tiles.assign(best.rec.tiles.begin(), best.rec.tiles.begin() + (long)k);
+      m.prefix.rsets.assign(best.rec.rsets.begin(), best.rec.rsets.begin() + (long)k);
+      m.use_prefix = true;
+    }
+    run(moves);
+    for (Try &m : moves)
+      if (better(m)) best = std::move(m);
+  }
+  if (chosen) *chosen = best.shape;
+  if (trace) *trace = best.rec;
+  return std::move(best.plan);
 }
 
 namespace {

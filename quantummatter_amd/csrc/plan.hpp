@@ -206,7 +206,12 @@ struct FusedShape {
   // With phase_search: gates a first phase's register set is charged for
   // holding a contiguous low bit (the load then needs a transpose).
   int low_penalty = 0;
+  // Tile search only (0: deterministic): seed of a perturbation of the tile
+  // growth, which adds a pseudo-random 0..kSearchNoise to each candidate
+  // qubit's count of runnable gates, so a restart explores other tiles.
+  uint64_t search_seed = 0;
 };
+constexpr int kSearchNoise = 2;
 
 struct FusedPass {
   DevPass dev;
@@ -240,6 +245,28 @@ struct PlanTrace {
 // decisions are taken; with `record`, the decisions are written there.
 FusedPlan plan_fused(const std::vector<DevGate> &gates, int n, const FusedShape &shape,
                      const PlanTrace *replay = nullptr, PlanTrace *record = nullptr);
+
+// Searched planning (compiled schedule).  The greedy tile search fills each
+// pass with the most runnable gates, which is myopic: on brick circuits the
+// tiles of one pass decide how much the next passes can run.  The search
+// (1) plans every base shape deterministically, (2) restarts the planner with
+// perturbed tile growth (FusedShape::search_seed) over the bases, and (3)
+// improves the best plan by iterated greedy moves: keep the best plan's first
+// k tiles (k drawn per move), replan the rest with a perturbed search, and
+// keep the result if it costs less.  Restarts and each round of moves run on
+// `threads` host threads (0: hardware threads, at most 16); every plan is
+// recorded, so the chosen plan replays exactly from its trace.  Returns the
+// plan of least `cost`; its shape and trace go to *chosen and *trace.
+struct PlanSearch {
+  int restarts = 48;   // perturbed restarts over the base shapes
+  int rounds = 6;      // rounds of iterated greedy moves
+  int moves = 16;      // moves per round (run in parallel)
+  int threads = 0;
+  uint64_t seed = 0x5EED;
+};
+FusedPlan plan_fused_searched(const std::vector<DevGate> &gates, int n, const std::vector<FusedShape> &bases,
+                              const PlanSearch &opt, double (*cost)(const FusedPlan &), FusedShape *chosen,
+                              PlanTrace *trace);
 
 // ---------------------------------------------------------------------------
 // Batched programs: one program run on many same-width states, each state by

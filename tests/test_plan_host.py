@@ -121,6 +121,22 @@ def test_replayed_plan_equals_fresh_plan(kind, n, shape):
     assert np.max(np.abs(got - exp)) <= 1e-13
 
 
+@pytest.mark.parametrize("kind,n", [("all", 13), ("all", 15), ("layer", 16), ("layer", 20)])
+@pytest.mark.parametrize("shape", [(13, 8, 3, 1, True), (12, 8, 4, 2, True)])
+def test_searched_plan_matches_oracle(kind, n, shape):
+    """The compiled schedule's searched plan (perturbed restarts and iterated
+    greedy moves over the tiles) replays from its recorded trace to the same
+    plan (checked in the library) and its state equals the oracle's."""
+    if n < shape[0]:
+        pytest.skip("register narrower than the tile")
+    c = random_all_kinds(n, 300, seed=n * 43 + shape[0]) if kind == "all" else \
+        workloads.random_layer_circuit(n, 16, seed=n + 5)
+    ops = c.to_ops()
+    exp = oracle.execute(n, ops)
+    got = plan_apply_host(n, ops, oracle.zero_state(n), *shape, searched=True)
+    assert np.max(np.abs(got - exp)) <= 1e-13
+
+
 def test_remapped_single_gates_are_exact_permutations():
     """A permutation gate under remapping moves amplitudes exactly."""
     n = 12
