@@ -422,6 +422,25 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           scan_tile(Q, false);
         }
       }
+      if (remap && deferred.empty() && displaced()) {
+        // Last pass: the qubits displaced from the low index bits must be in
+        // the tile for the store to restore the natural order (else a pass
+        // without gates follows just for that).  Tile qubits that no pair gate
+        // of the pass targets give them their slots; every gate still runs.
+        uint64_t need = forced, want = 0;
+        for (int idx : included)
+          if (G[idx].pair) need |= (uint64_t)1 << G[idx].g.target;
+        for (int p = 0; p < c; p++)
+          if (logi[p] != p) want |= (uint64_t)1 << p;
+        uint64_t missing = want & ~Q, spare = Q & ~need & ~want;
+        while (missing && spare) {
+          Q &= ~(spare & (~spare + 1));
+          Q |= missing & (~missing + 1);
+          spare &= spare - 1;
+          missing &= missing - 1;
+        }
+        scan_tile(Q, false);
+      }
     }
     if (record) {
       record->tiles.push_back(Q);
