@@ -112,7 +112,8 @@ struct BatchArena {
   int64_t slots = 0;
   size_t o_states = 0, o_items = 0, o_passes = 0, o_ops = 0, o_segs = 0, o_pools = 0, o_angles = 0,
          o_groups = 0, o_numer = 0, o_mass = 0, o_shotu = 0, o_samples = 0, o_outcomes = 0,
-         o_consumed = 0, o_err = 0, o_fb = 0;
+         o_consumed = 0, o_err = 0, o_fb = 0, o_part = 0, o_part_mass = 0;
+  int32_t slices = 0;  // > 0: partial sums of launch_pauli_slices (set before setup)
 
   // Sizes the arena for L (workgroup slots within the budget unless L.keep),
   // allocates it and uploads the inputs on the context's stream.
@@ -144,6 +145,8 @@ struct BatchArena {
     o_consumed = ar.add((size_t)n_items * 8);
     o_err = ar.add((size_t)n_items * 4);
     o_fb = ar.add(8);
+    o_part = ar.add(slices ? (size_t)(std::max<int64_t>(L.n_numer, 1) * slices) * 8 : 0);
+    o_part_mass = ar.add(slices ? (size_t)(n_items * slices) * 8 : 0);
     if ((c->device_budget && c->resident_bytes + (int64_t)ar.size > c->device_budget) ||
         c->resident_bytes + (int64_t)ar.size > c->device_memory)
       fail(QM_ERR_BUDGET, "HIP batch exceeds the device memory budget.");
@@ -194,9 +197,61 @@ struct BatchArena {
   }
 };
 
+// Slices per state when a launch's mass and Pauli numerators are better
+// reduced after the batch kernel by launch_pauli_slices, else 0.  Taken when
+// every item has the same width, of at least QM_PAULI_SLICE_QUBITS (tuning
+// knob, default 15; 0 turns it off) qubits, and the same Pauli groups, and
+// when all states can stay resident for the reduction.  One workgroup per
+// state would read its state from HBM once per flip group with every state of
+// the launch in flight; slicing each state over many workgroups keeps the few
+// states being reduced in the last-level cache.  A slice is at least 2048
+// pairs (8 per thread), at most 64 slices per state.
+static int pauli_slices(const Ctx *c, const BatchLaunch &L) {
+  static const int min_qubits = [] {
+    const char *e = std::getenv("QM_PAULI_SLICE_QUBITS");
+    return e ? std::atoi(e) : 15;
+  }();
+  if (min_qubits <= 0 || L.keep || L.items.empty() || L.groups.empty()) return 0;
+  const BatchItem &f = L.items.front();
+  if (f.group_count <= 0 || f.len < (1ull << min_qubits)) return 0;
+  for (const BatchItem &it : L.items)
+    if (it.len != f.len || it.group_begin != f.group_begin || it.group_count != f.group_count) return 0;
+  const int64_t n_items = (int64_t)L.items.size();
+  const int slices = (int)std::min<uint64_t>(64, std::max<uint64_t>(1, f.len / 2 / 2048));
+  // Resident states plus partials within the budget (the slot arena is the fallback).
+  const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
+  const int64_t need = n_items * (int64_t)f.len * amp_bytes(L.precision) +
+                       (std::max<int64_t>(L.n_numer, 1) + n_items) * slices * 8;
+  if (need > limit - c->resident_bytes - (256ll << 20)) return 0;
+  return slices;
+}
+
+// Turns L into the sliced form when pauli_slices takes it: states kept, the
+// batch kernel's own mass and numerators off (each item's group count and
+// mass flag zeroed; the groups they had go to *group_begin, *group_count).
+static int make_sliced(const Ctx *c, BatchLaunch &L, int32_t *group_begin, int32_t *group_count) {
+  const int slices = pauli_slices(c, L);
+  if (!slices) return 0;
+  *group_begin = L.items.front().group_begin;
+  *group_count = L.items.front().group_count;
+  L.keep = true;
+  L.keep_len = 0;
+  L.on_states = nullptr;
+  for (BatchItem &it : L.items) {
+    it.state_off = L.keep_len;
+    L.keep_len += (int64_t)it.len;
+    it.group_count = 0;
+    it.want_mass = 0;
+  }
+  return slices;
+}
+
 void run_batch_launch(Ctx *c, BatchLaunch &L) {
   const int64_t n_items = (int64_t)L.items.size();
+  int32_t gb = 0, gc = 0;
+  const int slices = make_sliced(c, L, &gb, &gc);
   BatchArena A;
+  A.slices = slices;
   A.setup(c, L);
   hipStream_t st = c->stream;
   QM_HIP(hipMemsetAsync(A.ar.at<char>(A.o_fb), 0, 8, st));
@@ -204,6 +259,9 @@ void run_batch_launch(Ctx *c, BatchLaunch &L) {
   int64_t grid = L.keep ? 0 : A.slots;
   if (L.timed) QM_HIP(hipEventRecord(c->ev_start, st));
   launch_batch(c, L.precision, L.shape, a, &grid);
+  if (slices)
+    launch_pauli_slices(c, L.precision, a.states, a.items, n_items, a.groups, gb, gc, slices,
+                        A.ar.at<double>(A.o_part), A.ar.at<double>(A.o_part_mass), a.numer, a.mass);
   if (L.timed) QM_HIP(hipEventRecord(c->ev_stop, st));
 
   L.err.assign((size_t)n_items, 0);
@@ -696,6 +754,8 @@ struct GraphedBatch {
   size_t off_numer = 0, off_mass = 0, off_fb = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int slices = 0;          // sliced mass and numerators (make_sliced), else 0
+  int32_t group_begin = 0, group_count = 0;
   int64_t held = 0;        // arena bytes counted in the context's resident bytes
   uint64_t last_use = 0;   // program tick of the last evaluation (cache eviction)
 
@@ -715,9 +775,12 @@ struct GraphedBatch {
                           hipMemcpyHostToDevice, st));
     k_zero_counter<<<1, 64, 0, st>>>(A.ar.at<unsigned long long>(A.o_fb));
     QM_HIP(hipGetLastError());
-    int64_t grid = A.slots;
+    int64_t grid = L.keep ? 0 : A.slots;
     launch_batch(c, L.precision, L.shape, a, &grid);
     const int64_t n = (int64_t)L.items.size();
+    if (slices)
+      launch_pauli_slices(c, L.precision, a.states, a.items, n, a.groups, group_begin, group_count, slices,
+                          A.ar.at<double>(A.o_part), A.ar.at<double>(A.o_part_mass), a.numer, a.mass);
     QM_HIP(hipMemcpyAsync(h_out, A.ar.at<char>(A.o_err), (size_t)n * 4, hipMemcpyDeviceToHost, st));
     QM_HIP(hipMemcpyAsync(h_out + off_numer, A.ar.at<char>(A.o_numer), (size_t)L.n_numer * 8,
                           hipMemcpyDeviceToHost, st));
@@ -729,6 +792,8 @@ struct GraphedBatch {
     c = ctx;
     job = std::move(j);
     g = build_launch(job, L);
+    slices = make_sliced(c, L, &group_begin, &group_count);
+    A.slices = slices;
     A.setup(c, L);
     // The arena stays allocated while the program lives: later state and batch
     // budget checks see it.

@@ -22,6 +22,7 @@
 #include "qm_internal.hpp"
 #include "tile_ops.hpp"
 
+
 namespace qm {
 
 namespace {
@@ -222,6 +223,44 @@ constexpr int kMaxAmbiguous = 64;
 
 }  // namespace
 
+// Mass and Pauli numerators of one item (StateVector.Observations.cs:107-141),
+// FP64, by the item's workgroup after its program.  Out of line: the program
+// loop's register tile is dead here, and this code's registers then do not add
+// to the program loop's (spills there cost every pass).
+template <typename T, int NT>
+__device__ __noinline__ void item_numerators(const T *__restrict__ s, uint64_t n, const BatchItem &it,
+                                             int64_t item, const PauliGroup *__restrict__ groups,
+                                             double *__restrict__ numer, double *__restrict__ mass,
+                                             double *sh_red) {
+  const uint32_t t = threadIdx.x;
+  double ms = 0;
+  for (uint64_t j = t; j < n; j += NT) ms += prob_of(s[j]);
+  ms = block_sum(ms, sh_red);
+  if (t == 0 && it.want_mass) mass[item] = ms;
+  for (int gi = it.group_begin; gi < it.group_begin + it.group_count; gi++) {
+    const PauliGroup &g = groups[gi];
+    double acc[kTermsPerGroup];
+    switch (g.count) {
+#define QM_GROUP_CASE(C)                                                                      \
+  case C: {                                                                                   \
+    double part[C];                                                                           \
+    group_numerators<T, C>(s, g, 0, g.flip == 0 ? n : n / 2, t, NT, part);                    \
+    for (int k = 0; k < C; k++) acc[k] = part[k];                                             \
+  } break;
+      QM_GROUP_CASE(1) QM_GROUP_CASE(2) QM_GROUP_CASE(3) QM_GROUP_CASE(4)
+      QM_GROUP_CASE(5) QM_GROUP_CASE(6) QM_GROUP_CASE(7) QM_GROUP_CASE(8)
+#undef QM_GROUP_CASE
+      default: break;
+    }
+    for (int k = 0; k < kTermsPerGroup; k++) {
+      if (k >= g.count) break;
+      const double r = block_sum(acc[k], sh_red);
+      if (t == 0) numer[it.numer_off + g.first + k] = r;
+    }
+  }
+  __syncthreads();
+}
+
 template <typename T, int L, int R>
 __global__ __launch_bounds__(1 << L, 2) void k_batch(const BatchArgs a) {
   using F = typename Real<T>::type;
@@ -310,61 +349,8 @@ __global__ __launch_bounds__(1 << L, 2) void k_batch(const BatchArgs a) {
     }
 
     // Mass and Pauli numerators (StateVector.Observations.cs:107-141), FP64.
-    if (!err && (it.want_mass || it.group_count > 0)) {
-      double ms = 0;
-      for (uint64_t j = t; j < n; j += NT) ms += prob_of(s[j]);
-      ms = block_sum(ms, sh_red);
-      if (t == 0 && it.want_mass) a.mass[item] = ms;
-      for (int gi = it.group_begin; gi < it.group_begin + it.group_count; gi++) {
-        const PauliGroup &g = a.groups[gi];
-        double acc[kTermsPerGroup];
-#pragma unroll
-        for (int k = 0; k < kTermsPerGroup; k++) acc[k] = 0;
-        if (g.flip == 0) {
-          for (uint64_t j = t; j < n; j += NT) {
-            const double p = prob_of(s[j]);
-#pragma unroll
-            for (int k = 0; k < kTermsPerGroup; k++) {
-              if (k < g.count) {
-                const int y = g.y[k] & 3;
-                double c = y == 0 ? p : y == 2 ? -p : 0.0;
-                if (__popcll(j & g.sign[k]) & 1) c = -c;
-                acc[k] += c;
-              }
-            }
-          }
-        } else {
-          // Each pair (j, j ^ flip) read once: the partner's overlap is the conjugate.
-          const int h = 63 - __clzll((long long)g.flip);
-          const uint64_t low = (1ull << h) - 1;
-          for (uint64_t q = t; q < n / 2; q += NT) {
-            const uint64_t j = ((q & ~low) << 1) | (q & low);
-            const uint64_t j2 = j ^ g.flip;
-            const T x = s[j], y = s[j2];
-            const double ar = x.re, ai = x.im, pr = y.re, pim = -(double)y.im;
-            const double ore = pr * ar - pim * ai;  // Complex.Conjugate(s[j2]) * s[j]
-            const double oim = pim * ar + pr * ai;
-#pragma unroll
-            for (int k = 0; k < kTermsPerGroup; k++) {
-              if (k < g.count) {
-                const int yk = g.y[k] & 3;
-                double c1 = yk == 0 ? ore : yk == 1 ? -oim : yk == 2 ? -ore : oim;
-                double c2 = yk == 0 ? ore : yk == 1 ? oim : yk == 2 ? -ore : -oim;
-                if (__popcll(j & g.sign[k]) & 1) c1 = -c1;
-                if (__popcll(j2 & g.sign[k]) & 1) c2 = -c2;
-                acc[k] += c1 + c2;
-              }
-            }
-          }
-        }
-        for (int k = 0; k < kTermsPerGroup; k++) {
-          if (k >= g.count) break;
-          const double r = block_sum(acc[k], sh_red);
-          if (t == 0) a.numer[it.numer_off + g.first + k] = r;
-        }
-      }
-      __syncthreads();
-    }
+    if (!err && (it.want_mass || it.group_count > 0))
+      item_numerators<T, NT>(s, n, it, item, a.groups, a.numer, a.mass, sh_red);
 
     // Terminal sampling (StateVectorSimulator.cs:44-90) and MeasureAll (StateVector.cs:106-153).
     if (!err && (it.shots > 0 || it.measure)) {
@@ -477,6 +463,88 @@ __global__ __launch_bounds__(1 << L, 2) void k_batch(const BatchArgs a) {
   }
 }
 
+// Mass and Pauli numerators of kept batch states (StateVector.Observations.cs:
+// 107-141, FP64), reduced after the batch kernel by many workgroups per state.
+// Work unit u = (item, slice) in item order, so the grid works on a few
+// consecutive states at a time and they stay in the device's last-level cache
+// across their reads (one per flip group), where one workgroup per state would
+// stream each state from HBM once per group with every state in flight.  Slice
+// sl of an item covers amplitudes [sl A, (sl + 1) A) for the mass and the flip
+// 0 group, and pair indices [sl P, (sl + 1) P) of every other group (A =
+// len / slices, P = len / 2 / slices), so every amplitude and every pair is
+// counted once.  Partial sums go to part[(numer_off + first + k) * slices + sl]
+// and part_mass[item * slices + sl]; k_pauli_fold adds them in slice order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pauli_slices(const T *__restrict__ states,
+                                                      const BatchItem *__restrict__ items, int64_t n_items,
+                                                      const PauliGroup *__restrict__ groups, int32_t group_begin,
+                                                      int32_t group_count, int32_t slices, double *__restrict__ part,
+                                                      double *__restrict__ part_mass) {
+  __shared__ double sh_red[16];
+  const uint32_t t = threadIdx.x, NT = blockDim.x;
+  const int64_t units = n_items * slices;
+  for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+    const int64_t item = u / slices;
+    const uint64_t sl = (uint64_t)(u % slices);
+    const BatchItem it = items[item];
+    const T *s = states + it.state_off;
+    const uint64_t n = it.len, A = n / (uint64_t)slices, P = A / 2;
+    double ms = 0;
+    for (uint64_t j = sl * A + t; j < (sl + 1) * A; j += NT) ms += prob_of(s[j]);
+    ms = block_sum(ms, sh_red);
+    if (t == 0) part_mass[u] = ms;
+    for (int gi = group_begin; gi < group_begin + group_count; gi++) {
+      const PauliGroup &g = groups[gi];
+      const uint64_t j0 = g.flip == 0 ? sl * A : sl * P, j1 = g.flip == 0 ? (sl + 1) * A : (sl + 1) * P;
+      double out[kTermsPerGroup];
+      switch (g.count) {
+#define QM_GROUP_CASE(C)                                                   \
+  case C: {                                                                \
+    double acc[C];                                                         \
+    group_numerators<T, C, 4>(s, g, j0, j1, t, NT, acc);                   \
+    for (int k = 0; k < C; k++) out[k] = acc[k];                           \
+  } break;
+        QM_GROUP_CASE(1) QM_GROUP_CASE(2) QM_GROUP_CASE(3) QM_GROUP_CASE(4)
+        QM_GROUP_CASE(5) QM_GROUP_CASE(6) QM_GROUP_CASE(7) QM_GROUP_CASE(8)
+#undef QM_GROUP_CASE
+        default: break;
+      }
+      for (int k = 0; k < kTermsPerGroup; k++) {
+        if (k >= g.count) break;
+        const double r = block_sum(out[k], sh_red);
+        if (t == 0) part[(it.numer_off + g.first + k) * slices + (int64_t)sl] = r;
+      }
+    }
+  }
+}
+
+// Sums of k_pauli_slices' partials in slice order: one thread per item (mass)
+// and per (item, group) (the group's numerators).
+__global__ void k_pauli_fold(const BatchItem *__restrict__ items, int64_t n_items,
+                             const PauliGroup *__restrict__ groups, int32_t group_begin, int32_t group_count,
+                             int32_t slices, const double *__restrict__ part,
+                             const double *__restrict__ part_mass, double *__restrict__ numer,
+                             double *__restrict__ mass) {
+  const int64_t total = n_items + n_items * group_count;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    if (x < n_items) {
+      double m = 0;
+      for (int s = 0; s < slices; s++) m += part_mass[x * slices + s];
+      mass[x] = m;
+      continue;
+    }
+    const int64_t y = x - n_items, item = y / group_count;
+    const PauliGroup &g = groups[group_begin + (int32_t)(y % group_count)];
+    for (int k = 0; k < g.count; k++) {
+      const int64_t idx = items[item].numer_off + g.first + k;
+      double v = 0;
+      for (int s = 0; s < slices; s++) v += part[idx * slices + s];
+      numer[idx] = v;
+    }
+  }
+}
+
 namespace {
 
 template <typename T, int L, int R>
@@ -529,6 +597,28 @@ void launch_batch(Ctx *c, int precision, const FusedShape &shape, const BatchArg
   QM_BATCH_SHAPES(QM_LAUNCH)
 #undef QM_LAUNCH
   fail(QM_ERR_UNSUPPORTED, "batch tile shape is not compiled");
+}
+
+void launch_pauli_slices(Ctx *c, int precision, const void *states, const BatchItem *items, int64_t n_items,
+                         const PauliGroup *groups, int32_t group_begin, int32_t group_count, int32_t slices,
+                         double *part, double *part_mass, double *numer, double *mass) {
+  // Two workgroups per CU: the grid is on about 2 x CUs / slices states at a
+  // time (8 states of 2^20 amplitudes, 128 MiB in complex128, at 64 slices).
+  const int64_t units = n_items * slices;
+  const unsigned blocks =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, (int64_t)c->compute_units * 2));
+  if (precision == QM_C128)
+    k_pauli_slices<C128><<<blocks, 256, 0, c->stream>>>(static_cast<const C128 *>(states), items, n_items, groups,
+                                                         group_begin, group_count, slices, part, part_mass);
+  else
+    k_pauli_slices<C64><<<blocks, 256, 0, c->stream>>>(static_cast<const C64 *>(states), items, n_items, groups,
+                                                        group_begin, group_count, slices, part, part_mass);
+  QM_HIP(hipGetLastError());
+  const int64_t total = n_items * (1 + (int64_t)group_count);
+  const unsigned fold_blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 1024));
+  k_pauli_fold<<<fold_blocks, 256, 0, c->stream>>>(items, n_items, groups, group_begin, group_count, slices, part,
+                                                    part_mass, numer, mass);
+  QM_HIP(hipGetLastError());
 }
 
 }  // namespace qm

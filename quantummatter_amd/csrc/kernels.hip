@@ -425,46 +425,18 @@ __global__ __launch_bounds__(kBlock) void k_pauli(const T *__restrict__ s, uint6
   const uint64_t begin = (uint64_t)blockIdx.x * per;
   const uint64_t end = begin + per < n ? begin + per : n;
   double acc[kTermsPerGroup];
-#pragma unroll
-  for (int k = 0; k < kTermsPerGroup; k++) acc[k] = 0;
-  if (g.flip == 0) {
-    for (uint64_t j = begin + threadIdx.x; j < end; j += kBlock) {
-      const double p = prob_of(s[j]);  // Complex.Conjugate(a) * a = |a|^2
-#pragma unroll
-      for (int k = 0; k < kTermsPerGroup; k++) {
-        if (k < g.count) {
-          const int y = g.y[k] & 3;
-          double c = y == 0 ? p : y == 2 ? -p : 0.0;
-          if (__popcll(j & g.sign[k]) & 1) c = -c;
-          acc[k] += c;
-        }
-      }
-    }
-  } else {
-    // Each pair (j, j ^ flip) read once: the partner's overlap is the conjugate.
-    const int h = 63 - __clzll((long long)g.flip);
-    const uint64_t low = (1ull << h) - 1;
-    for (uint64_t q = begin / 2 + threadIdx.x; q < end / 2; q += kBlock) {
-      const uint64_t j = ((q & ~low) << 1) | (q & low);
-      const uint64_t j2 = j ^ g.flip;
-      const T a = s[j];
-      const T p = s[j2];
-      const double ar = a.re, ai = a.im, pr = p.re, pim = -(double)p.im;
-      // Complex.Conjugate(s[j2]) * s[j]; its conjugate is the term at j2.
-      const double ore = pr * ar - pim * ai;
-      const double oim = pim * ar + pr * ai;
-#pragma unroll
-      for (int k = 0; k < kTermsPerGroup; k++) {
-        if (k < g.count) {
-          const int y = g.y[k] & 3;
-          double c1 = y == 0 ? ore : y == 1 ? -oim : y == 2 ? -ore : oim;
-          double c2 = y == 0 ? ore : y == 1 ? oim : y == 2 ? -ore : -oim;
-          if (__popcll(j & g.sign[k]) & 1) c1 = -c1;
-          if (__popcll(j2 & g.sign[k]) & 1) c2 = -c2;
-          acc[k] += c1 + c2;
-        }
-      }
-    }
+  switch (g.count) {
+#define QM_GROUP_CASE(C)                                                                              \
+  case C: {                                                                                           \
+    double part[C];                                                                                   \
+    group_numerators<T, C>(s, g, g.flip == 0 ? begin : begin / 2, g.flip == 0 ? end : end / 2,        \
+                           threadIdx.x, kBlock, part);                                                \
+    for (int k = 0; k < C; k++) acc[k] = part[k];                                                     \
+  } break;
+    QM_GROUP_CASE(1) QM_GROUP_CASE(2) QM_GROUP_CASE(3) QM_GROUP_CASE(4)
+    QM_GROUP_CASE(5) QM_GROUP_CASE(6) QM_GROUP_CASE(7) QM_GROUP_CASE(8)
+#undef QM_GROUP_CASE
+    default: break;
   }
   for (int k = 0; k < kTermsPerGroup; k++) {
     if (k >= g.count) break;
