@@ -136,6 +136,7 @@ class QuantumCircuit:
             raise ArgumentOutOfRangeError("qubits", f"Number of qubits must be between 1 and {MAX_QUBITS}.")
         self.QubitCount = int(qubits)
         self._ops: List[GateOperation] = []
+        self._packed = None  # (qm_op records, operation count they were formed at)
 
     @property
     def Operations(self) -> Sequence[GateOperation]:
@@ -218,21 +219,22 @@ class QuantumCircuit:
         return self._add(GateKind.Toffoli, target, first, second=second)
 
     def to_ops(self) -> np.ndarray:
-        return ops_array(self._ops)
+        """The operations in qm_op layout.  Operations are only ever appended, so
+        the records are formed once per operation count (callers must not
+        modify the array)."""
+        if self._packed is None or self._packed[1] != len(self._ops):
+            self._packed = (ops_array(self._ops), len(self._ops))
+        return self._packed[0]
+
+
+_NO_UNITARY = (0.0,) * 8
 
 
 def ops_array(ops: Iterable[GateOperation]) -> np.ndarray:
-    ops = list(ops)
-    a = np.zeros(len(ops), dtype=OP_DTYPE)
-    for i, op in enumerate(ops):
-        a[i]["kind"] = int(op.Kind)
-        a[i]["target"] = op.Target
-        a[i]["control"] = op.Control
-        a[i]["second_control"] = op.SecondControl
-        a[i]["angle"] = op.Angle
-        if op.Unitary is not None:
-            a[i]["unitary"] = op.Unitary.packed()
-    return a
+    """The operations in qm_op layout (one record per operation, built in one pass)."""
+    return np.array([(int(op.Kind), op.Target, op.Control, op.SecondControl, op.Angle,
+                      _NO_UNITARY if op.Unitary is None else tuple(op.Unitary.packed()))
+                     for op in ops], dtype=OP_DTYPE)
 
 
 class PauliString:
@@ -288,19 +290,19 @@ class PauliHamiltonian:
                 raise ArgumentError("Every term must be nonnull and match the register size.")
         self.QubitCount = qubits
         self.Terms = tuple(terms)
+        self._packed = None
 
     def packed(self):
-        return pauli_array([t.Observable for t in self.Terms]), \
-            np.array([t.Coefficient for t in self.Terms], dtype=np.float64)
+        """(qm_pauli records, coefficients) of the terms; the terms are fixed, so
+        the arrays are formed once (callers must not modify them)."""
+        if self._packed is None:
+            self._packed = (pauli_array([t.Observable for t in self.Terms]),
+                            np.array([t.Coefficient for t in self.Terms], dtype=np.float64))
+        return self._packed
 
 
 def pauli_array(strings: Sequence[PauliString]) -> np.ndarray:
-    a = np.zeros(len(strings), dtype=PAULI_DTYPE)
-    for i, p in enumerate(strings):
-        a[i]["flip_mask"] = p.FlipMask
-        a[i]["sign_mask"] = p.SignMask
-        a[i]["y_count"] = p.YCount
-    return a
+    return np.array([(p.FlipMask, p.SignMask, p.YCount, 0) for p in strings], dtype=PAULI_DTYPE)
 
 
 class NoiseKind(enum.IntEnum):
@@ -393,6 +395,7 @@ class ParameterizedCircuit:
         self._ops: List[GateOperation] = []
         self._param_index: List[int] = []
         self._rotations: List[int] = []
+        self._packed = None  # (ops, param_index, operation count they were formed at)
 
     @property
     def OperationCount(self) -> int:
@@ -453,8 +456,11 @@ class ParameterizedCircuit:
         return out
 
     def packed(self):
-        """Returns (template ops, parameter index per op) in qm_op / int32 layout."""
-        return ops_array(self._ops), np.asarray(self._param_index, dtype=np.int32)
+        """Returns (template ops, parameter index per op) in qm_op / int32 layout,
+        formed once per template state (callers must not modify them)."""
+        if self._packed is None or self._packed[2] != len(self._ops):
+            self._packed = (ops_array(self._ops), np.asarray(self._param_index, dtype=np.int32), len(self._ops))
+        return self._packed[0], self._packed[1]
 
 
 class DynamicCircuit:
