@@ -204,12 +204,19 @@ struct BatchArena {
 // when all states can stay resident for the reduction.  One workgroup per
 // state would read its state from HBM once per flip group with every state of
 // the launch in flight; slicing each state over many workgroups keeps the few
-// states being reduced in the last-level cache.  A slice is at least 2048
-// pairs (8 per thread), at most 64 slices per state.
-static int pauli_slices(const Ctx *c, const BatchLaunch &L) {
+// states being reduced in the last-level cache.  With QM_PAULI_TILES (tuning
+// knob, default 1) and the terms' LDS slots within 32 KiB, a slice is a tile
+// of 2^kPauliTileBitsHost amplitudes and the tile-pair kernel shares each read
+// among the groups of one partner tile (*tile_pairs set); else a slice is at
+// least 2048 pairs (8 per thread), at most 64 slices per state.
+static int pauli_slices(const Ctx *c, const BatchLaunch &L, bool *tile_pairs, int32_t *n_terms) {
   static const int min_qubits = [] {
     const char *e = std::getenv("QM_PAULI_SLICE_QUBITS");
     return e ? std::atoi(e) : 15;
+  }();
+  static const bool tiles_on = [] {
+    const char *e = std::getenv("QM_PAULI_TILES");
+    return e ? std::atoi(e) != 0 : true;
   }();
   if (min_qubits <= 0 || L.keep || L.items.empty() || L.groups.empty()) return 0;
   const BatchItem &f = L.items.front();
@@ -217,7 +224,13 @@ static int pauli_slices(const Ctx *c, const BatchLaunch &L) {
   for (const BatchItem &it : L.items)
     if (it.len != f.len || it.group_begin != f.group_begin || it.group_count != f.group_count) return 0;
   const int64_t n_items = (int64_t)L.items.size();
-  const int slices = (int)std::min<uint64_t>(64, std::max<uint64_t>(1, f.len / 2 / 2048));
+  int32_t terms = 0;
+  for (int32_t k = f.group_begin; k < f.group_begin + f.group_count; k++)
+    terms = std::max(terms, L.groups[(size_t)k].first + L.groups[(size_t)k].count);
+  *n_terms = terms;
+  *tile_pairs = tiles_on && f.len >= (1ull << kPauliTileBitsHost) && (int64_t)terms * 4 * 8 <= (32 << 10);
+  const int slices = *tile_pairs ? (int)(f.len >> kPauliTileBitsHost)
+                                 : (int)std::min<uint64_t>(64, std::max<uint64_t>(1, f.len / 2 / 2048));
   // Resident states plus partials within the budget (the slot arena is the fallback).
   const int64_t limit = c->device_budget ? c->device_budget : c->device_memory;
   const int64_t need = n_items * (int64_t)f.len * amp_bytes(L.precision) +
@@ -229,8 +242,9 @@ static int pauli_slices(const Ctx *c, const BatchLaunch &L) {
 // Turns L into the sliced form when pauli_slices takes it: states kept, the
 // batch kernel's own mass and numerators off (each item's group count and
 // mass flag zeroed; the groups they had go to *group_begin, *group_count).
-static int make_sliced(const Ctx *c, BatchLaunch &L, int32_t *group_begin, int32_t *group_count) {
-  const int slices = pauli_slices(c, L);
+static int make_sliced(const Ctx *c, BatchLaunch &L, int32_t *group_begin, int32_t *group_count,
+                       bool *tile_pairs, int32_t *n_terms) {
+  const int slices = pauli_slices(c, L, tile_pairs, n_terms);
   if (!slices) return 0;
   *group_begin = L.items.front().group_begin;
   *group_count = L.items.front().group_count;
@@ -248,8 +262,9 @@ static int make_sliced(const Ctx *c, BatchLaunch &L, int32_t *group_begin, int32
 
 void run_batch_launch(Ctx *c, BatchLaunch &L) {
   const int64_t n_items = (int64_t)L.items.size();
-  int32_t gb = 0, gc = 0;
-  const int slices = make_sliced(c, L, &gb, &gc);
+  int32_t gb = 0, gc = 0, n_terms = 0;
+  bool tile_pairs = false;
+  const int slices = make_sliced(c, L, &gb, &gc, &tile_pairs, &n_terms);
   BatchArena A;
   A.slices = slices;
   A.setup(c, L);
@@ -260,7 +275,7 @@ void run_batch_launch(Ctx *c, BatchLaunch &L) {
   if (L.timed) QM_HIP(hipEventRecord(c->ev_start, st));
   launch_batch(c, L.precision, L.shape, a, &grid);
   if (slices)
-    launch_pauli_slices(c, L.precision, a.states, a.items, n_items, a.groups, gb, gc, slices,
+    launch_pauli_slices(c, L.precision, a.states, a.items, n_items, tile_pairs, a.groups, gb, gc, n_terms, slices,
                         A.ar.at<double>(A.o_part), A.ar.at<double>(A.o_part_mass), a.numer, a.mass);
   if (L.timed) QM_HIP(hipEventRecord(c->ev_stop, st));
 
@@ -755,7 +770,8 @@ struct GraphedBatch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int slices = 0;          // sliced mass and numerators (make_sliced), else 0
-  int32_t group_begin = 0, group_count = 0;
+  int32_t group_begin = 0, group_count = 0, n_terms = 0;
+  bool tile_pairs = false;  // tile-pair reduction (pauli_slices)
   int64_t held = 0;        // arena bytes counted in the context's resident bytes
   uint64_t last_use = 0;   // program tick of the last evaluation (cache eviction)
 
@@ -779,7 +795,8 @@ struct GraphedBatch {
     launch_batch(c, L.precision, L.shape, a, &grid);
     const int64_t n = (int64_t)L.items.size();
     if (slices)
-      launch_pauli_slices(c, L.precision, a.states, a.items, n, a.groups, group_begin, group_count, slices,
+      launch_pauli_slices(c, L.precision, a.states, a.items, n, tile_pairs, a.groups, group_begin, group_count,
+                          n_terms, slices,
                           A.ar.at<double>(A.o_part), A.ar.at<double>(A.o_part_mass), a.numer, a.mass);
     QM_HIP(hipMemcpyAsync(h_out, A.ar.at<char>(A.o_err), (size_t)n * 4, hipMemcpyDeviceToHost, st));
     QM_HIP(hipMemcpyAsync(h_out + off_numer, A.ar.at<char>(A.o_numer), (size_t)L.n_numer * 8,
@@ -792,7 +809,7 @@ struct GraphedBatch {
     c = ctx;
     job = std::move(j);
     g = build_launch(job, L);
-    slices = make_sliced(c, L, &group_begin, &group_count);
+    slices = make_sliced(c, L, &group_begin, &group_count, &tile_pairs, &n_terms);
     A.slices = slices;
     A.setup(c, L);
     // The arena stays allocated while the program lives: later state and batch

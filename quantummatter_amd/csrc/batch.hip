@@ -518,6 +518,163 @@ __global__ __launch_bounds__(256) void k_pauli_slices(const T *__restrict__ stat
   }
 }
 
+// Mass and Pauli numerators of kept batch states by tile pairs.  A state of
+// 2^n amplitudes is 2^(n - kPauliTileBits) tiles; a pair (j, j ^ flip) joins
+// tile h = j >> kPauliTileBits with tile h ^ v, v = flip >> kPauliTileBits.
+// Work unit (item, tile h): the unit holds tile h in registers (lane t holds
+// offsets t + 256 r, r < 16) and, for every group with h < h ^ v, or v = 0,
+// adds the pairs whose lower tile is h (for v = 0 the pairs inside the tile
+// whose top flip bit is 0), reading the partner amplitudes, all 16 of a lane
+// at once.  Groups are sorted by flip mask, so the groups of one v are
+// consecutive and share the partner tile in cache.  Sign parities split into
+// a tile part (uniform), a register part (uniform per r) and a lane part
+// applied once per group; exactly one of a term's Re/Im weights is nonzero,
+// so a term costs two multiply-adds per pair.  Per-wave sums of each term go
+// to LDS slots [term][wave] and are added in wave order at the end of the
+// unit into part[(numer_off + term) * tiles + h]; k_pauli_fold adds the tiles
+// in order.  Terms of the groups a unit skips get 0 from it.  Units are
+// ordered XCD-major when the grid is a multiple of 8 and there are at least
+// 8 items: the workgroups of one XCD (block id mod 8) reduce items x, x + 8,
+// ... so a state's partner tiles are shared through that XCD's L2.
+constexpr int kPauliTileBits = kPauliTileBitsHost;
+constexpr int kPauliTileThreads = 256;
+constexpr int kPauliTileRegs = (1 << kPauliTileBits) / kPauliTileThreads;
+static_assert(kPauliTileThreads == 256 && kPauliTileRegs == 16, "lane/register split below assumes 8 + 4 bits");
+
+struct TileHome {
+  double re[kPauliTileRegs], im[kPauliTileRegs];
+};
+
+// One group's per-lane numerators over tile h (C = the group's term count).
+template <typename T, int C>
+__device__ __forceinline__ void tile_group(const T *__restrict__ s, uint64_t h, const PauliGroup &g, uint32_t t,
+                                           const TileHome &a, double *__restrict__ lds_terms, uint32_t lane,
+                                           uint32_t w) {
+  constexpr int RG = kPauliTileRegs;
+  const uint64_t hb = h << kPauliTileBits;
+  double ar[C], ai[C], cr[C], ci[C], sl[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    ar[k] = 0;
+    ai[k] = 0;
+    const int y = g.y[k] & 3;
+    const double alpha = y == 0 ? 1.0 : y == 2 ? -1.0 : 0.0, beta = y == 1 ? -1.0 : y == 3 ? 1.0 : 0.0;
+    if (g.flip == 0) {
+      cr[k] = alpha;  // |a_j|^2 with the term's Y factor
+      ci[k] = 0;
+    } else {
+      const bool odd = (__popcll(g.flip & g.sign[k]) & 1) != 0;
+      cr[k] = odd ? 0.0 : 2.0 * alpha;
+      ci[k] = odd ? 2.0 * beta : 0.0;
+    }
+    // Tile and lane parts of the term's sign parity at j = hb | t | r << 8.
+    const int par = (__popcll(hb & g.sign[k]) + __popc(t & (uint32_t)g.sign[k] & 0xFFu)) & 1;
+    sl[k] = par ? -1.0 : 1.0;
+  }
+  if (g.flip == 0) {
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+      const double p = a.re[r] * a.re[r] + a.im[r] * a.im[r];
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        const double sg = (__popc(((uint32_t)r << 8) & (uint32_t)g.sign[k]) & 1) ? -1.0 : 1.0;
+        ar[k] = fma(sg, p, ar[k]);
+      }
+    }
+  } else {
+    const uint64_t v = g.flip >> kPauliTileBits;
+    const uint32_t fl = (uint32_t)g.flip & ((1u << kPauliTileBits) - 1);
+    const uint32_t fr = fl >> 8;
+    const T *ps = s + (((h ^ v) << kPauliTileBits) | (t ^ (fl & 0xFFu)));
+    // v = 0: pairs inside the tile, counted from the member whose top flip
+    // bit is 0 (a register condition when that bit is >= 8, else a lane one).
+    const uint32_t top = v == 0 ? 1u << (31 - __clz(fl)) : 0u;
+    T b[RG];
+#pragma unroll
+    for (int r = 0; r < RG; r++) b[r] = ps[(uint32_t)(r ^ fr) << 8];
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+      if (((uint32_t)r << 8) & top) continue;  // uniform
+      const double br = b[r].re, bi = b[r].im;
+      const double ore = fma(br, a.re[r], bi * a.im[r]);  // Complex.Conjugate(s[j ^ flip]) * s[j]
+      const double oim = fma(br, a.im[r], -(bi * a.re[r]));
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        const double sg = (__popc(((uint32_t)r << 8) & (uint32_t)g.sign[k]) & 1) ? -1.0 : 1.0;
+        ar[k] = fma(sg, ore, ar[k]);
+        ai[k] = fma(sg, oim, ai[k]);
+      }
+    }
+    if (t & top) {
+#pragma unroll
+      for (int k = 0; k < C; k++) sl[k] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < C; k++) {
+    const double x = wave_sum(sl[k] * (cr[k] * ar[k] + ci[k] * ai[k]));
+    if (lane == 0) lds_terms[(g.first + k) * 4 + w] = x;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPauliTileThreads, 2) void k_pauli_tiles(
+    const T *__restrict__ states, const BatchItem *__restrict__ items, int64_t n_items,
+    const PauliGroup *__restrict__ groups, int32_t group_begin, int32_t group_count, int32_t n_terms,
+    double *__restrict__ part, double *__restrict__ part_mass) {
+  constexpr int NT = kPauliTileThreads, RG = kPauliTileRegs, NX = 8;
+  extern __shared__ double lds_terms[];  // [n_terms][4]
+  __shared__ double sh_mass[4];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint64_t tiles = items[0].len >> kPauliTileBits;
+  const bool xm = gridDim.x % NX == 0 && n_items >= NX;
+  const int64_t x = xm ? blockIdx.x % NX : 0, slot = xm ? blockIdx.x / NX : blockIdx.x;
+  const int64_t stride = xm ? gridDim.x / NX : gridDim.x;
+  const int64_t my_items = xm ? (n_items - x + NX - 1) / NX : n_items;
+  for (int64_t k = slot; k < my_items * (int64_t)tiles; k += stride) {
+    const int64_t item = xm ? x + NX * (k / (int64_t)tiles) : k / (int64_t)tiles;
+    const uint64_t h = (uint64_t)(k % (int64_t)tiles);
+    const BatchItem it = items[item];
+    const T *s = states + it.state_off;
+    const uint64_t hb = h << kPauliTileBits;
+    __syncthreads();  // the previous unit's LDS slots are read
+    for (int i = (int)t; i < n_terms * 4; i += NT) lds_terms[i] = 0;
+    TileHome a;
+    double ms = 0;
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+      const T v = s[hb | (t + (uint64_t)NT * r)];
+      a.re[r] = v.re;
+      a.im[r] = v.im;
+      ms += a.re[r] * a.re[r] + a.im[r] * a.im[r];
+    }
+    ms = wave_sum(ms);
+    if (lane == 0) sh_mass[w] = ms;
+    __syncthreads();  // slots cleared
+    for (int gi = group_begin; gi < group_begin + group_count; gi++) {
+      const PauliGroup &g = groups[gi];
+      const uint64_t v = g.flip >> kPauliTileBits;
+      if (v != 0 && (h ^ v) < h) continue;  // the pair tile's unit adds these pairs
+      switch (g.count) {
+#define QM_TILE_CASE(C) \
+  case C:               \
+    tile_group<T, C>(s, h, g, t, a, lds_terms, lane, w); \
+    break;
+        QM_TILE_CASE(1) QM_TILE_CASE(2) QM_TILE_CASE(3) QM_TILE_CASE(4)
+        QM_TILE_CASE(5) QM_TILE_CASE(6) QM_TILE_CASE(7) QM_TILE_CASE(8)
+#undef QM_TILE_CASE
+        default: break;
+      }
+    }
+    __syncthreads();
+    const int64_t u = item * (int64_t)tiles + (int64_t)h;
+    for (int i = (int)t; i < n_terms; i += NT)
+      part[(it.numer_off + i) * (int64_t)tiles + (int64_t)h] =
+          ((lds_terms[i * 4] + lds_terms[i * 4 + 1]) + lds_terms[i * 4 + 2]) + lds_terms[i * 4 + 3];
+    if (t == 0) part_mass[u] = ((sh_mass[0] + sh_mass[1]) + sh_mass[2]) + sh_mass[3];
+  }
+}
+
 // Sums of k_pauli_slices' partials in slice order: one thread per item (mass)
 // and per (item, group) (the group's numerators).
 __global__ void k_pauli_fold(const BatchItem *__restrict__ items, int64_t n_items,
@@ -600,14 +757,24 @@ void launch_batch(Ctx *c, int precision, const FusedShape &shape, const BatchArg
 }
 
 void launch_pauli_slices(Ctx *c, int precision, const void *states, const BatchItem *items, int64_t n_items,
-                         const PauliGroup *groups, int32_t group_begin, int32_t group_count, int32_t slices,
-                         double *part, double *part_mass, double *numer, double *mass) {
+                         bool tile_pairs, const PauliGroup *groups, int32_t group_begin, int32_t group_count,
+                         int32_t n_terms, int32_t slices, double *part, double *part_mass, double *numer,
+                         double *mass) {
   // Two workgroups per CU: the grid is on about 2 x CUs / slices states at a
   // time (8 states of 2^20 amplitudes, 128 MiB in complex128, at 64 slices).
   const int64_t units = n_items * slices;
   const unsigned blocks =
       (unsigned)std::max<int64_t>(1, std::min<int64_t>(units, (int64_t)c->compute_units * 2));
-  if (precision == QM_C128)
+  if (tile_pairs) {
+    // Tile pairs (k_pauli_tiles): slices = tiles of 2^13 amplitudes.
+    const size_t lds = (size_t)n_terms * 4 * sizeof(double);
+    if (precision == QM_C128)
+      k_pauli_tiles<C128><<<blocks, 256, lds, c->stream>>>(static_cast<const C128 *>(states), items, n_items, groups,
+                                                           group_begin, group_count, n_terms, part, part_mass);
+    else
+      k_pauli_tiles<C64><<<blocks, 256, lds, c->stream>>>(static_cast<const C64 *>(states), items, n_items, groups,
+                                                          group_begin, group_count, n_terms, part, part_mass);
+  } else if (precision == QM_C128)
     k_pauli_slices<C128><<<blocks, 256, 0, c->stream>>>(static_cast<const C128 *>(states), items, n_items, groups,
                                                          group_begin, group_count, slices, part, part_mass);
   else

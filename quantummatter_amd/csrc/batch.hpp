@@ -104,11 +104,16 @@ void launch_batch(Ctx *c, int precision, const FusedShape &shape, const BatchArg
                   int64_t *grid);
 
 // Mass and Pauli numerators of kept states of one width (items' state_off),
-// many workgroups per state (k_pauli_slices), then the partial sums added in
-// slice order (k_pauli_fold) into mass[item] and numer[numer_off + first + k].
-// part holds n_numer x slices doubles, part_mass n_items x slices.
+// many workgroups per state, then the partial sums added in slice order
+// (k_pauli_fold) into mass[item] and numer[numer_off + first + k].  With
+// tile_pairs, slices = len / 2^kPauliTileBitsHost tile units (k_pauli_tiles,
+// n_terms * 32 bytes of LDS), else len / slices-amplitude slices
+// (k_pauli_slices).  part holds n_numer x slices doubles, part_mass
+// n_items x slices.
+constexpr int kPauliTileBitsHost = 12;
 void launch_pauli_slices(Ctx *c, int precision, const void *states, const BatchItem *items, int64_t n_items,
-                         const PauliGroup *groups, int32_t group_begin, int32_t group_count, int32_t slices,
-                         double *part, double *part_mass, double *numer, double *mass);
+                         bool tile_pairs, const PauliGroup *groups, int32_t group_begin, int32_t group_count,
+                         int32_t n_terms, int32_t slices, double *part, double *part_mass, double *numer,
+                         double *mass);
 
 }  // namespace qm

@@ -126,6 +126,47 @@ def test_energy_gradient_batch_c128(n, layers, terms):
     assert np.max(np.abs(np.array(res.Gradient) - g_want)) <= 1e-11 * scale
 
 
+def _structured_hamiltonian(n, seed):
+    """Terms of every shape the after-batch Pauli reductions split on: Z-only
+    and identity strings (no flip), flips inside the lowest 12 qubits only,
+    flips reaching the top qubits, Y factors, and more than 8 terms on one
+    flip mask (split over several groups of that mask)."""
+    rng = np.random.default_rng(seed)
+
+    def term(labels):  # labels: qubit -> 'X' | 'Y' | 'Z'
+        s = "".join(labels.get(q, "I") for q in reversed(range(n)))
+        return PauliTerm(float(rng.normal()), PauliString(s))
+
+    terms = [term({}), term({0: "Z", n - 1: "Z"}), term({3: "Z", 7: "Z", 11: "Z"})]
+    for lo in range(6):  # flips in the low 12 qubits
+        terms.append(term({lo: "X", lo + 5: "Y"}))
+        terms.append(term({lo: "Y", lo + 3: "Z"}))
+    for q in (12, n - 2, n - 1):  # flips with high bits, alone and with low ones
+        terms.append(term({q: "X"}))
+        terms.append(term({q: "Y", 2: "X", 9: "Z"}))
+    zs = [{1: "Z"}, {4: "Z"}, {n - 1: "Z"}, {6: "Z", 13: "Z"}, {}, {0: "Z"}, {8: "Z"},
+          {10: "Z", 2: "Z"}, {5: "Z"}, {n - 3: "Z"}, {11: "Z"}]
+    for extra in zs:  # eleven terms on one flip mask (qubits 2 and n - 1)
+        terms.append(term({2: "X", n - 1: "Y", **{k: v for k, v in extra.items() if k not in (2, n - 1)}}))
+    return PauliHamiltonian(n, terms)
+
+
+@pytest.mark.parametrize("precision", [C128, C64])
+def test_energy_gradient_after_batch_pauli_reduction(precision):
+    """15 qubits (at the width where mass and numerators are reduced after
+    the batch kernel) with a Hamiltonian of every group shape."""
+    n = 15
+    circuit = workloads.hardware_efficient_ansatz(n, 1)
+    theta = workloads.random_parameters(circuit.ParameterCount, 41)
+    ham = _structured_hamiltonian(n, 5)
+    e_want, g_want = oracle_gradient(circuit, theta, ham)
+    scale = sum(abs(t.Coefficient) for t in ham.Terms)
+    tol = (1e-11 if precision == C128 else 1e-4) * scale
+    res = HipBatchedEnergyEvaluator(shared_backend(), precision).EvaluateWithGradient(circuit, ham, theta)
+    assert abs(res.Energy - e_want) <= tol
+    assert np.max(np.abs(np.array(res.Gradient) - g_want)) <= tol
+
+
 def test_energy_gradient_batch_c64():
     circuit, theta, ham = hea_case(12, 2, 64, seed=3)
     e_want, g_want = oracle_gradient(circuit, theta, ham)
