@@ -285,24 +285,33 @@ FusedPlan plan_cached(const std::vector<DevGate> &gates, int n, int precision, c
 // pass-time model fitted on nine plans of the 30-qubit c128 layer circuit at
 // the default 8192-amplitude tiles with two workgroups per CU (0.077 ms per
 // gate, the same in every plan, + 1.8 ms per pass + 0.26 ms per LDS exchange
-// over a 16 GiB state; within 2.5 %).  Search budget knobs: QM_PLAN_RESTARTS
-// (perturbed restarts, default 96; 0 plans the base shapes only) and
-// QM_PLAN_ROUNDS (rounds of 16 iterated greedy moves, default 8).
+// over a 16 GiB state; within 2.5 %).  Search budget: 96 perturbed restarts
+// and 8 rounds of 16 iterated greedy moves below 28 qubits; from 28 qubits,
+// where one pass of a state costs ~2.5 ms or more and the plan is cached per
+// structure, 1000 restarts and 60 rounds (30-qubit 40-layer layer circuit:
+// 17 -> 16 passes, 226 -> 237 gate-layers/s, ~1 s of planning on 16 host
+// threads).  Knobs: QM_PLAN_RESTARTS (perturbed restarts; 0 plans the base
+// shapes only) and QM_PLAN_ROUNDS (rounds of moves) override both budgets.
 double compiled_plan_cost(const FusedPlan &p) {
   double t = 0;
   for (const FusedPass &x : p.passes) t += 1.8 + 0.26 * x.transposes;
   return t;
 }
 FusedPlan plan_compiled(const std::vector<DevGate> &gates, int n, const FusedShape &shape, const std::string &key) {
-  static const PlanSearch opt = [] {
-    PlanSearch o;
-    o.restarts = 96;
-    o.rounds = 8;
-    if (const char *e = std::getenv("QM_PLAN_RESTARTS")) o.restarts = std::max(0, std::atoi(e));
-    if (const char *e = std::getenv("QM_PLAN_ROUNDS")) o.rounds = std::max(0, std::atoi(e));
-    if (o.restarts == 0) o.rounds = 0;
-    return o;
+  static const int env_restarts = [] {
+    const char *e = std::getenv("QM_PLAN_RESTARTS");
+    return e ? std::max(0, std::atoi(e)) : -1;
   }();
+  static const int env_rounds = [] {
+    const char *e = std::getenv("QM_PLAN_ROUNDS");
+    return e ? std::max(0, std::atoi(e)) : -1;
+  }();
+  PlanSearch opt;
+  opt.restarts = n >= 28 ? 1000 : 96;
+  opt.rounds = n >= 28 ? 60 : 8;
+  if (env_restarts >= 0) opt.restarts = env_restarts;
+  if (env_rounds >= 0) opt.rounds = env_rounds;
+  if (opt.restarts == 0) opt.rounds = 0;
   std::vector<int> lows{shape.contiguous}, caps{shape.max_phases};
   if (shape.alt_contiguous > 0 && shape.alt_contiguous < shape.K && shape.alt_contiguous != shape.contiguous)
     lows.push_back(shape.alt_contiguous);
