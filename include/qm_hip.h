@@ -288,7 +288,11 @@ int qm_plan_fused(int32_t qubits, int32_t precision, const qm_op *ops, int64_t n
  * coefficients, must equal a fresh plan, and is the one applied), bit 4 the
  * compiled schedule's searched planning (perturbed restarts and iterated
  * greedy moves over the tile choices, a small budget; the plan replayed from
- * its recorded trace must equal the searched plan, and is the one applied). */
+ * its recorded trace must equal the searched plan, and is the one applied),
+ * bit 5 the lane layout (the plan is replayed from its trace with every pass
+ * storing under the index-bit permutation that puts the next pass's load
+ * lanes above the contiguous run; passes then alternate between the state and
+ * a second buffer, as the compiled schedule runs them). */
 int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t tile_bits,
                        int32_t lane_bits, int32_t contiguous, int32_t flags, double *amplitudes);
 

@@ -455,14 +455,17 @@ def dense_apply_host(n: int, ops: np.ndarray, state: np.ndarray, max_block_qubit
 
 def plan_apply_host(qubits: int, ops: np.ndarray, state: np.ndarray, tile_bits: int, lane_bits: int,
                     contiguous: int, remap: int = 0, search: bool = False, replay: bool = False,
-                    searched: bool = False) -> np.ndarray:
+                    searched: bool = False, lane_layout: bool = False) -> np.ndarray:
     """Runs the fused schedule on the host with the kernels' arithmetic (CPU verification).
     remap: remapping mode of the low index bits (0, 1 or 2); search: tile search;
     replay: the plan is replayed from decisions recorded on the same gates with
     other coefficients (and checked against a fresh plan); searched: the
     compiled schedule's searched planning (restarts and iterated greedy moves),
-    replayed from its trace (and checked against the searched plan)."""
+    replayed from its trace (and checked against the searched plan);
+    lane_layout: every pass stores under the index-bit permutation that puts
+    the next pass's load lanes above the contiguous run (passes out of place)."""
     s = np.array(state, dtype=np.complex128, copy=True)
     call("qm_plan_apply_host", qubits, ptr(ops), len(ops), tile_bits, lane_bits, contiguous,
-         int(remap) | (4 if search else 0) | (8 if replay else 0) | (16 if searched else 0), ptr(s))
+         int(remap) | (4 if search else 0) | (8 if replay else 0) | (16 if searched else 0)
+         | (32 if lane_layout else 0), ptr(s))
     return s

@@ -194,7 +194,8 @@ std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n) {
 // (lowest index bits kept in every tile), QM_REMAP (remapping mode 0..2),
 // QM_SEARCH (tile search 0/1), QM_MAX_PHASES (phases per pass at most),
 // QM_PHASE_SEARCH (register sets of the phases 0..2) and QM_PHASE_LOW_PENALTY
-// (charge, in gates, of a contiguous low bit in a pass's first register set).
+// (charge, in gates, of a contiguous low bit in a pass's first register set)
+// and QM_LANE_LAYOUT (0 turns the compiled schedule's lane layout off).
 void apply_tuning_knobs(FusedShape &shape) {
   auto knob = [](const char *name) {
     const char *e = std::getenv(name);
@@ -202,7 +203,8 @@ void apply_tuning_knobs(FusedShape &shape) {
   };
   static const int contiguous = knob("QM_CONTIGUOUS"), remap = knob("QM_REMAP"), search = knob("QM_SEARCH"),
                    max_phases = knob("QM_MAX_PHASES"), phase_search = knob("QM_PHASE_SEARCH"),
-                   low_penalty = knob("QM_PHASE_LOW_PENALTY");
+                   low_penalty = knob("QM_PHASE_LOW_PENALTY"), lane_layout = knob("QM_LANE_LAYOUT");
+  if (lane_layout == 0 || lane_layout == 1) shape.lane_layout = shape.lane_layout && lane_layout;
   if (phase_search >= 0 && phase_search <= 2) shape.phase_search = phase_search;
   if (low_penalty >= 0) shape.low_penalty = low_penalty;
   if (contiguous >= 0 && contiguous < shape.K) shape.contiguous = contiguous, shape.alt_contiguous = 0;
@@ -239,7 +241,7 @@ std::string plan_key(const std::vector<DevGate> &gates, int n, int precision, co
   auto put = [&](const void *p, size_t bytes) { k.append((const char *)p, bytes); };
   const int32_t head[] = {n,           precision,      schedule,         sh.K,         sh.L,
                           sh.R,        sh.contiguous,  sh.remap,         sh.search,    sh.max_phases,
-                          sh.alt_contiguous, sh.phase_search, sh.low_penalty};
+                          sh.alt_contiguous, sh.phase_search, sh.low_penalty, sh.lane_layout};
   put(head, sizeof head);
   k.reserve(k.size() + gates.size() * 28);
   for (const DevGate &g : gates) {
@@ -327,6 +329,9 @@ FusedPlan plan_compiled(const std::vector<DevGate> &gates, int n, const FusedSha
   if (bases.empty()) bases.push_back(shape);
   CachedPlan rec{shape, {}};
   FusedPlan plan = plan_fused_searched(gates, n, bases, opt, compiled_plan_cost, &rec.shape, &rec.trace);
+  // The lane layout needs each next pass's load lanes: the chosen plan is
+  // replayed from its trace, which holds them (same tiles, phases and gates).
+  if (rec.shape.lane_layout) plan = plan_fused(gates, n, rec.shape, &rec.trace);
   plan_cache_put(key, std::move(rec));
   return plan;
 }
@@ -366,6 +371,7 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
       shape.remap = wide.remap;
       shape.search = wide.search;
       shape.max_phases = wide.max_phases;
+      shape.lane_layout = wide.lane_layout;
     }
     if (!force && sched == QM_SCHED_FUSED && !(tuned_slot >= 0 && c->tuned[0][s->precision][s->qubits].K)) {
       // The interpreting kernel keeps the lowest qubits in place (its passes are
@@ -375,6 +381,8 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
       shape.remap = 0;
       shape.search = 1;
     }
+    // The interpreting kernel runs every pass in place.
+    if (sched == QM_SCHED_FUSED) shape.lane_layout = 0;
     if (!force) apply_tuning_knobs(shape);
     return shape;
   };
@@ -416,14 +424,35 @@ ExecStats run_gates(State *s, const std::vector<DevGate> &gates, int schedule_in
         if (!ready && auto_compile()) jit_prepare_async(plan, s->precision, c->device);
       }
     }
+    bool out_place = false;
+    for (const FusedPass &p : plan.passes) out_place = out_place || p.dev.out_place;
+    // Passes that store under another lane layout than they load read one
+    // buffer and write the other; without room for the second buffer this
+    // call runs interpreted.
+    if (ready && out_place && !state_scratch(s)) ready = false;
     if (ready) {
       const int64_t bytes = (int64_t)(jp.coef.size() * sizeof(double));
       double *d_coef = (double *)ctx_program(c, bytes + 256);
       if (bytes)
         QM_HIP(hipMemcpyAsync(d_coef, jp.coef.data(), (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+      // The buffer holding the newest state becomes the state's own, also
+      // when a cancellation ends the run between passes.
+      struct Current {
+        State *s;
+        void *cur;
+        ~Current() {
+          if (cur != s->data) std::swap(s->data, s->scratch);
+        }
+      } cur{s, s->data};
       for (size_t i = 0; i < plan.passes.size(); i++) {
         check_cancel(cancel);
-        jit_launch(s, shape, jp, i, d_coef, c->stream);
+        if (plan.passes[i].dev.out_place) {
+          void *other = cur.cur == s->data ? s->scratch : s->data;
+          jit_launch(s, shape, jp, i, d_coef, c->stream, cur.cur, other);
+          cur.cur = other;
+        } else {
+          jit_launch(s, shape, jp, i, d_coef, c->stream, cur.cur, cur.cur);
+        }
         st.launches++;
         st.passes++;
         st.bytes += 2 * state_bytes;
@@ -669,7 +698,26 @@ void state_free(State *s) {
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(s->data);
   c->resident_bytes -= s->bytes;
+  if (s->scratch) {
+    (void)hipFree(s->scratch);
+    c->resident_bytes -= s->bytes;
+  }
   delete s;
+}
+
+bool state_scratch(State *s) {
+  if (s->scratch) return true;
+  Ctx *c = s->ctx;
+  if ((c->device_budget && c->resident_bytes + s->bytes > c->device_budget) ||
+      c->resident_bytes + s->bytes > c->device_memory)
+    return false;
+  if (hipMalloc(&s->scratch, (size_t)s->bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    s->scratch = nullptr;
+    return false;
+  }
+  c->resident_bytes += s->bytes;
+  return true;
 }
 
 }  // namespace qm
@@ -1153,8 +1201,22 @@ int qm_jit_compile_check(int32_t qubits, int32_t precision, const qm_op *ops, in
     if (qubits < tile_bits || qubits > 34) fail(QM_ERR_RANGE, "qubits");
     std::vector<DevGate> gates = prepare_all(ops, n_ops, qubits);
     FusedShape shape = compiled_shape(tile_bits, lane_bits);
+    // Debugging aid QM_JIT_CHECK_SEARCHED=1: the plan the compiled schedule
+    // runs (the context default shape when tile_bits and lane_bits are the
+    // default's, searched planning), instead of the greedy plan of the shape.
+    static const bool searched = [] {
+      const char *e = std::getenv("QM_JIT_CHECK_SEARCHED");
+      return e && std::atoi(e) != 0;
+    }();
+    if (searched) {
+      const Ctx defaults;
+      const FusedShape &d = defaults.shape[precision];
+      if (d.K == tile_bits && d.L == lane_bits) shape = d;
+    }
     apply_tuning_knobs(shape);
-    FusedPlan plan = plan_fused(gates, qubits, shape);
+    FusedPlan plan = searched ? plan_compiled(gates, qubits, shape,
+                                              plan_key(gates, qubits, precision, shape, QM_SCHED_COMPILED))
+                              : plan_fused(gates, qubits, shape);
     *passes = jit_compile_check(plan, precision);
   });
 }
@@ -1311,7 +1373,8 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
     FusedShape shape{tile_bits, lane_bits, tile_bits - lane_bits, contiguous};
     shape.remap = flags & 3;
     shape.search = (flags >> 2) & 1;
-    if (flags < 0 || flags > 31 || shape.remap == 3) fail(QM_ERR_RANGE, "unsupported planning flags");
+    shape.lane_layout = (flags >> 5) & 1;
+    if (flags < 0 || flags > 63 || shape.remap == 3) fail(QM_ERR_RANGE, "unsupported planning flags");
     if (shape.R < 1 || shape.R > 5 || shape.L < 1 || shape.L > 10 || shape.K > kMaxK ||
         contiguous < 0 || contiguous >= shape.K)
       fail(QM_ERR_RANGE, "unsupported tile shape");
@@ -1335,8 +1398,14 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
       bool same = replayed.passes.size() == searched.passes.size();
       for (size_t i = 0; same && i < searched.passes.size(); i++) {
         const FusedPass &a = replayed.passes[i], &b = searched.passes[i];
-        same = std::memcmp(&a.dev, &b.dev, sizeof(DevPass)) == 0 && a.ops.size() == b.ops.size() &&
-               (a.ops.empty() || std::memcmp(a.ops.data(), b.ops.data(), a.ops.size() * sizeof(TileOp)) == 0);
+        if (chosen.lane_layout) {
+          // The replay adds the lane layout: the same tiles, phases and gates.
+          same = a.rsets == b.rsets && a.gates == b.gates && a.ops.size() == b.ops.size() &&
+                 a.dev.qmask == b.dev.qmask && a.dev.out_place == 1;
+        } else {
+          same = std::memcmp(&a.dev, &b.dev, sizeof(DevPass)) == 0 && a.ops.size() == b.ops.size() &&
+                 (a.ops.empty() || std::memcmp(a.ops.data(), b.ops.data(), a.ops.size() * sizeof(TileOp)) == 0);
+        }
       }
       if (!same) fail(QM_ERR_DEVICE, "a replayed searched plan differs from the searched plan");
       apply_plan_host(replayed, amps, qubits);
@@ -1349,10 +1418,11 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
       std::vector<DevGate> other = g;
       for (DevGate &x : other)
         for (double &v : x.c) v = 0.5 * v + 0.25;
-      PlanTrace trace;
+      PlanTrace trace, fresh_trace;
       (void)plan_fused(other, qubits, shape, nullptr, &trace);
       FusedPlan replayed = plan_fused(g, qubits, shape, &trace);
-      FusedPlan fresh = plan_fused(g, qubits, shape);
+      FusedPlan fresh = plan_fused(g, qubits, shape, nullptr, &fresh_trace);
+      if (shape.lane_layout) fresh = plan_fused(g, qubits, shape, &fresh_trace);
       bool same = replayed.passes.size() == fresh.passes.size();
       for (size_t i = 0; same && i < fresh.passes.size(); i++) {
         const FusedPass &a = replayed.passes[i], &b = fresh.passes[i];
@@ -1363,7 +1433,9 @@ int qm_plan_apply_host(int32_t qubits, const qm_op *ops, int64_t n_ops, int32_t 
       apply_plan_host(replayed, amps, qubits);
       return;
     }
-    FusedPlan plan = plan_fused(g, qubits, shape);
+    PlanTrace trace;
+    FusedPlan plan = plan_fused(g, qubits, shape, nullptr, &trace);
+    if (shape.lane_layout) plan = plan_fused(g, qubits, shape, &trace);
     apply_plan_host(plan, amps, qubits);
   });
 }

@@ -449,9 +449,14 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   // each of the CU's four SIMDs.
   (void)occ3;
   o << "extern \"C\" __global__ __launch_bounds__(" << (1 << L) << ", (QM_OCC * " << (1 << L)
-    << " + 255) / 256) void qm_pass(" << T
-    << " *__restrict__ s, const double *__restrict__ " << (tile_coef ? "cf_arg" : "cf")
-    << ", unsigned long long tiles) {\n";
+    << " + 255) / 256) void qm_pass(";
+  // A pass of a lane-layout plan reads one buffer (s) and writes the other
+  // (so): its tiles store to other index bits than they load from.
+  if (d.out_place)
+    o << "const " << T << " *__restrict__ s, " << T << " *__restrict__ so";
+  else
+    o << T << " *__restrict__ s";
+  o << ", const double *__restrict__ " << (tile_coef ? "cf_arg" : "cf") << ", unsigned long long tiles) {\n";
   o << "  typedef " << F << " F;\n";
   if (split)
     o << "  extern __shared__ F lds[];\n";
@@ -530,6 +535,13 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     }
   }
   o << "    const unsigned long long base = " << base_expr("tau") << ";\n";
+  if (d.store_nruns) {
+    std::string e = "0ull";
+    for (int k = 0; k < d.store_nruns; k++)
+      e += " | (((tau >> " + std::to_string((int)d.store_run_src[k]) + ") & " +
+           hex((1ull << d.store_run_len[k]) - 1) + ") << " + std::to_string((int)d.store_run_dst[k]) + ")";
+    o << "    const unsigned long long sbase = " << e << ";\n";
+  }
   o << "    F vr[" << N << "], vi[" << N << "];\n";
   if (prefetch) {
     for (int r = 0; r < N; r++) o << "    vr[" << r << "] = nr[" << r << "]; vi[" << r << "] = ni[" << r << "];\n";
@@ -1095,7 +1107,8 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
         }
     }
   }
-  o << "    { " << T << " *dst = s + (base | (sl" << flips << "));\n";
+  o << "    { " << T << " *dst = " << (d.out_place ? "so" : "s") << " + (" << (d.store_nruns ? "sbase" : "base")
+    << " | (sl" << flips << "));\n";
   const int nf = (int)rflips.size();
   for (int m = 0; nf && m < (1 << nf); m++) {
     o << "      " << T << " *const dst" << m << " = dst";
@@ -1440,17 +1453,17 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
 }
 
 void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,
-                const double *d_coef, hipStream_t st) {
+                const double *d_coef, hipStream_t st, const void *src, void *dst) {
   const uint64_t tiles = 1ull << (s->qubits - shape.K);
   const size_t lds = jit_lds_bytes(s->precision, shape);
   const uint64_t want = (uint64_t)s->ctx->compute_units * (uint64_t)jp.resident[pass];
   const unsigned blocks = (unsigned)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
-  void *state = s->data;
   const double *cf = d_coef + jp.coef_offset[pass];
   unsigned long long nt = tiles;
-  void *args[] = {&state, &cf, &nt};
+  void *args_in[] = {&dst, &cf, &nt};
+  void *args_out[] = {&src, &dst, &cf, &nt};
   QM_HIP(hipModuleLaunchKernel((hipFunction_t)jp.fns[pass], blocks, 1, 1, 1u << shape.L, 1, 1,
-                               (unsigned)lds, st, args, nullptr));
+                               (unsigned)lds, st, src == dst ? args_in : args_out, nullptr));
 }
 
 int jit_compile_check(const FusedPlan &plan, int precision) {

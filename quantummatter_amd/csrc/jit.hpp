@@ -34,8 +34,10 @@ void jit_prepare_async(const FusedPlan &plan, int precision, int device);
 void jit_plan_async(const std::string &structure, std::function<FusedPlan()> make, int precision, int device);
 // Waits until no background compilation is running.
 void jit_drain();
+// Launches pass `pass` of a plan on state s: in place (src == dst, a pass of
+// a plan without lane layout) or from src to dst (DevPass::out_place).
 void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,
-                const double *d_coef, hipStream_t st);
+                const double *d_coef, hipStream_t st, const void *src, void *dst);
 void jit_stats(int64_t *compiled, int64_t *hits, double *seconds);
 // Generates and compiles every pass kernel without loading it (no device needed).
 int jit_compile_check(const FusedPlan &plan, int precision);

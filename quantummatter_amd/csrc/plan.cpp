@@ -298,6 +298,11 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
   // (a transposition); every other bit holds its own qubit.
   int logi[64];
   for (int x = 0; x < 64; x++) logi[x] = x;
+  // Lane layout (FusedShape::lane_layout): the index bit each bit of the
+  // assignment above is stored at (identity below `contiguous`), for the
+  // state as the current pass loads it.
+  int mload[64];
+  for (int x = 0; x < 64; x++) mload[x] = x;
   auto displaced = [&] {
     for (int p = 0; p < c; p++)
       if (logi[p] != p) return true;
@@ -472,16 +477,23 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           k++;
         }
     }
-    // Runs of non-tile bits, ascending.
+    // Runs of non-tile bits, ascending (index bits under the load's lane
+    // layout); tile index bit k goes to the k-th lowest non-tile index bit.
+    int nt_virt[64], n_nt = 0;  // non-tile bits (before the lane layout) in tile index bit order
     {
+      uint64_t Pl = 0;
+      for (int x = 0; x < n; x++)
+        if (P & ((uint64_t)1 << x)) Pl |= (uint64_t)1 << mload[x];
+      int virt_at[64];
+      for (int x = 0; x < 64; x++) virt_at[mload[x]] = x;
       int src = 0, nr = 0;
       for (int q = 0; q < n;) {
-        if (P & ((uint64_t)1 << q)) {
+        if (Pl & ((uint64_t)1 << q)) {
           q++;
           continue;
         }
         int start = q;
-        while (q < n && !(P & ((uint64_t)1 << q))) q++;
+        while (q < n && !(Pl & ((uint64_t)1 << q))) nt_virt[n_nt++] = virt_at[q++];
         d.run_src[nr] = (uint8_t)src;
         d.run_dst[nr] = (uint8_t)start;
         d.run_len[nr] = (uint8_t)(q - start);
@@ -794,9 +806,69 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         lane_off[k] = (uint64_t)1 << lane_gbit[k];
       }
     };
-    set_io(d.load_direct ? 0 : kIo, qpos, d.load_reg_off, d.load_lane_gbit, d.load_lane_off);
-    set_io(d.store_direct ? d.nphases - 1 : kIoStore, store_pos, d.store_reg_off, d.store_lane_gbit,
-           d.store_lane_off);
+    // Lane layout of the store: the next pass's load lanes contiguous..+2
+    // (recorded in the replayed trace) to index bits contiguous..+2, this
+    // pass's store lanes contiguous..+2 to the next three free bits, every
+    // other bit keeping its order; the natural order when no next pass is
+    // known (the last pass, or planning without a trace).
+    const int st_slot = d.store_direct ? d.nphases - 1 : kIoStore;
+    int mstore[64];
+    for (int x = 0; x < 64; x++) mstore[x] = x;
+    d.out_place = sh.lane_layout && replaying && pass_index < replay->lanes.size() ? 1 : 0;
+    if (d.out_place && pass_index + 1 < replay->lanes.size()) {
+      bool assigned[64] = {}, taken[64] = {};
+      for (int x = 0; x < c; x++) assigned[x] = taken[x] = true;
+      const uint32_t nl = replay->lanes[pass_index + 1];
+      for (int k = 0; k < 3; k++) {
+        const int q = (int)((nl >> (8 * k)) & 0xffu);
+        if (q >= n || c + k >= n) continue;
+        const int v = newphys[q];
+        if (v < c || assigned[v] || taken[c + k]) continue;
+        mstore[v] = c + k;
+        assigned[v] = taken[c + k] = true;
+      }
+      int next = c + 3;
+      for (int k = c; k < c + 3 && k < L; k++) {
+        const int v = store_pos[d.lanepos[st_slot][k]];
+        if (assigned[v]) continue;
+        while (next < n && taken[next]) next++;
+        if (next >= std::min(n, c + 6)) break;
+        mstore[v] = next;
+        assigned[v] = taken[next] = true;
+      }
+      int free_phys = c;
+      for (int v = c; v < n; v++) {
+        if (assigned[v]) continue;
+        while (taken[free_phys]) free_phys++;
+        mstore[v] = free_phys;
+        taken[free_phys] = true;
+      }
+    }
+    int load_phys[16], store_phys[16];
+    for (int k = 0; k < K; k++) {
+      load_phys[k] = mload[qpos[k]];
+      store_phys[k] = mstore[store_pos[k]];
+    }
+    set_io(d.load_direct ? 0 : kIo, load_phys, d.load_reg_off, d.load_lane_gbit, d.load_lane_off);
+    set_io(st_slot, store_phys, d.store_reg_off, d.store_lane_gbit, d.store_lane_off);
+    // Store runs: tile index bit k (the non-tile bit nt_virt[k]) to its index
+    // bit under the store's lane layout (non-tile bits keep their place in
+    // the assignment, only the lane layout moves them).
+    {
+      bool same = true;
+      for (int k = 0; k < n_nt; k++) same = same && mstore[nt_virt[k]] == mload[nt_virt[k]];
+      int nr = 0;
+      for (int k = 0; k < n_nt && !same;) {
+        const int start = k;
+        while (k + 1 < n_nt && mstore[nt_virt[k + 1]] == mstore[nt_virt[k]] + 1) k++;
+        k++;
+        d.store_run_src[nr] = (uint8_t)start;
+        d.store_run_dst[nr] = (uint8_t)mstore[nt_virt[start]];
+        d.store_run_len[nr] = (uint8_t)(k - start);
+        nr++;
+      }
+      d.store_nruns = nr;
+    }
 
     std::vector<uint64_t> op_touch, op_pair_target;
     std::vector<TileOp> tail;  // merged diagonal ops, kept after every phase's ops
@@ -817,7 +889,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
           if (!(x.g.ctrl_mask & ((uint64_t)1 << q))) continue;
           int lp = local_of[q];
           if (lp < 0)
-            t.out_cmask |= (uint64_t)1 << phys[q];
+            t.out_cmask |= (uint64_t)1 << mload[phys[q]];
           else if (rset & (1u << lp))
             t.reg_cmask |= 1u << rank_in(rset, lp);
           else
@@ -827,7 +899,7 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         if (x.pair) {
           t.rbit = rank_in(rset, lp);
         } else if (lp < 0) {
-          t.dout = (uint64_t)1 << phys[x.g.target];
+          t.dout = (uint64_t)1 << mload[phys[x.g.target]];
         } else if (rset & (1u << lp)) {
           t.dreg = 1u << rank_in(rset, lp);
         } else {
@@ -860,9 +932,21 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
       if (d.phase_diag_count[ph]) d.phase_diag_begin[ph] += tail_base;
     pass.ops.insert(pass.ops.end(), tail.begin(), tail.end());
     pass.rsets = rsets;
+    if (record) {
+      // Load thread bits contiguous..contiguous+2 of this pass, as qubits.
+      const int ld_slot = d.load_direct ? 0 : kIo;
+      uint32_t lanes = 0xffffffu;
+      for (int k = 0; k < 3; k++)
+        if (c + k < L) {
+          const uint32_t q = (uint32_t)logi[qpos[d.lanepos[ld_slot][c + k]]];
+          lanes = (lanes & ~(0xffu << (8 * k))) | (q << (8 * k));
+        }
+      record->lanes.push_back(lanes);
+    }
     plan.passes.push_back(std::move(pass));
     pending.swap(deferred);
     for (int x = 0; x < 64; x++) logi[x] = newlogi[x];
+    for (int x = 0; x < 64; x++) mload[x] = mstore[x];
   }
   return plan;
 }
@@ -1192,18 +1276,34 @@ void apply_gate_host(const DevGate &g, double *s, int n) {
   }
 }
 
-void apply_plan_host(const FusedPlan &plan, double *s, int n) {
+void apply_plan_host(const FusedPlan &plan, double *state, int n) {
   const int K = plan.shape.K, L = plan.shape.L, R = plan.shape.R;
   const int regs = 1 << R, threads = 1 << L;
   std::vector<double> lds(2 * ((size_t)1 << K));
   std::vector<double> v(2 * (size_t)regs * threads);
+  // Out-of-place passes (lane layout) alternate between the state and a
+  // second buffer, as on the device.
+  std::vector<double> other;
+  double *s = state;
   for (const FusedPass &p : plan.passes) {
     const DevPass &d = p.dev;
+    double *out = s;
+    if (d.out_place) {
+      if (other.empty()) other.resize(2 * ((size_t)1 << n));
+      out = s == state ? other.data() : state;
+    }
     const uint64_t tiles = (uint64_t)1 << (n - K);
     for (uint64_t tau = 0; tau < tiles; tau++) {
       uint64_t base = 0;
       for (int r = 0; r < d.nruns; r++)
         base |= ((tau >> d.run_src[r]) & (((uint64_t)1 << d.run_len[r]) - 1)) << d.run_dst[r];
+      uint64_t sbase = base;
+      if (d.store_nruns) {
+        sbase = 0;
+        for (int r = 0; r < d.store_nruns; r++)
+          sbase |= ((tau >> d.store_run_src[r]) & (((uint64_t)1 << d.store_run_len[r]) - 1))
+                   << d.store_run_dst[r];
+      }
       auto lane_local = [&](int slot, int t) {
         uint32_t j = 0;
         for (int k = 0; k < L; k++)
@@ -1281,13 +1381,15 @@ void apply_plan_host(const FusedPlan &plan, double *s, int n) {
         for (int k = 0; k < L; k++)
           if ((t >> k) & 1) lo |= (uint64_t)1 << d.store_lane_gbit[k];
         for (int r = 0; r < regs; r++) {
-          uint64_t g = base | lo | d.store_reg_off[r];
-          s[2 * g] = v[2 * (t * regs + r)];
-          s[2 * g + 1] = v[2 * (t * regs + r) + 1];
+          uint64_t g = sbase | lo | d.store_reg_off[r];
+          out[2 * g] = v[2 * (t * regs + r)];
+          out[2 * g + 1] = v[2 * (t * regs + r) + 1];
         }
       }
     }
+    s = out;
   }
+  if (s != state) std::memcpy(state, s, sizeof(double) * 2 * ((size_t)1 << n));
 }
 
 }  // namespace qm

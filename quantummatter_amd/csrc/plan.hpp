@@ -136,7 +136,15 @@ struct DevPass {
   uint8_t run_src[16];
   uint8_t run_dst[16];
   uint8_t run_len[16];
-  uint8_t qpos[16];  // tile index bits (physical), ascending
+  uint8_t qpos[16];  // tile index bits (before the lane layout), ascending
+  // Runs of the store's tile index -> global base when the pass stores under
+  // another lane layout than it loaded with (FusedShape::lane_layout);
+  // store_nruns == 0: the load's runs.
+  int32_t store_nruns;
+  int32_t out_place;  // 1: the pass reads one state buffer and writes another (lane layout)
+  uint8_t store_run_src[32];
+  uint8_t store_run_dst[32];
+  uint8_t store_run_len[32];
 };
 
 // LDS address swizzle: XOR of the low 3 bits of a 16-byte-slot index with a
@@ -210,6 +218,16 @@ struct FusedShape {
   // growth, which adds a pseudo-random 0..kSearchNoise to each candidate
   // qubit's count of runnable gates, so a restart explores other tiles.
   uint64_t search_seed = 0;
+  // Lane layout (1: on; takes effect when a plan is replayed from a trace
+  // that holds each pass's load lanes).  Every pass stores the state under a
+  // permutation of the index bits above the `contiguous` low bits chosen so
+  // that the next pass's thread bits contiguous..contiguous+2 (the lane bits
+  // of a wave above one contiguous run) load from index bits
+  // contiguous..contiguous+2 and this pass's own store lanes go to the next
+  // three bits: each load and store instruction of a wave then covers
+  // 2^(contiguous+3) consecutive amplitudes or a few nearby runs instead of
+  // eight runs far apart.  The last pass stores in natural order.
+  int lane_layout = 0;
 };
 constexpr int kSearchNoise = 2;
 
@@ -238,6 +256,9 @@ struct FusedPlan {
 struct PlanTrace {
   std::vector<uint64_t> tiles;
   std::vector<std::vector<uint32_t>> rsets;
+  // Qubits of each pass's load thread bits contiguous..contiguous+2, 8 bits
+  // each (0xff: none), for the lane layout of a replay.
+  std::vector<uint32_t> lanes;
 };
 
 // Plans prepared gates (identity removed) for an n-qubit state, n >= shape.K.

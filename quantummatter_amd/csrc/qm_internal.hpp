@@ -56,11 +56,13 @@ int guarded(Fn &&fn) {
 
 // Default form of a compiled-schedule shape: 2^K-amplitude tiles over 2^L
 // threads, the lowest `contiguous` index bits (default 4) in every tile,
-// remapped between passes, tiles chosen by the tile search.
+// remapped between passes, tiles chosen by the tile search, lane layout on
+// (searched plans run out of place between the state and a scratch buffer).
 inline FusedShape compiled_shape(int K, int L, int contiguous = 4) {
   FusedShape s{K, L, K - L, contiguous};
   s.remap = 1;
   s.search = 1;
+  s.lane_layout = 1;
   return s;
 }
 
@@ -118,6 +120,10 @@ struct State {
   void *data = nullptr;
   int64_t bytes = 0;
   int64_t boundary_fallbacks = 0;
+  // Second buffer of `bytes` for passes that read one buffer and write the
+  // other (lane-layout plans); allocated on first use, counted in the
+  // context's resident bytes, freed with the state.
+  void *scratch = nullptr;
 };
 
 inline int64_t amp_bytes(int precision) { return precision == QM_C128 ? 16 : 8; }
@@ -130,6 +136,9 @@ std::vector<DevGate> prepare_all(const qm_op *ops, int64_t n_ops, int n);  // va
 // Resident states and gate execution (api.hip).
 State *state_new(Ctx *c, int qubits, int precision);  // checks the context's budget
 void state_free(State *s);
+// Allocates the state's scratch buffer if it has none; false when it does not fit
+// the budget or the device.
+bool state_scratch(State *s);
 struct ExecStats {
   int launches = 0;
   int passes = 0;

@@ -137,6 +137,27 @@ def test_searched_plan_matches_oracle(kind, n, shape):
     assert np.max(np.abs(got - exp)) <= 1e-13
 
 
+@pytest.mark.parametrize("kind,n", [("all", 13), ("all", 15), ("layer", 18)])
+@pytest.mark.parametrize("shape", [(13, 8, 3, 1, True), (12, 7, 4, 2, True)])
+@pytest.mark.parametrize("how", ["plain", "replay", "searched"])
+def test_lane_layout_plan_matches_oracle(kind, n, shape, how):
+    """Lane layout: each pass stores under the index-bit permutation that puts
+    the next pass's load lanes right above the contiguous run (tiles then
+    store to other index bits than they load, so passes alternate between
+    the state and a second buffer).  The state after the last pass, stored in
+    natural order, equals the oracle's; replayed and searched plans are
+    checked in the library as without the layout."""
+    if n < shape[0]:
+        pytest.skip("register narrower than the tile")
+    c = random_all_kinds(n, 300, seed=n * 47 + shape[0]) if kind == "all" else \
+        workloads.random_layer_circuit(n, 16, seed=n + 9)
+    ops = c.to_ops()
+    exp = oracle.execute(n, ops)
+    got = plan_apply_host(n, ops, oracle.zero_state(n), *shape, lane_layout=True,
+                          replay=how == "replay", searched=how == "searched")
+    assert np.max(np.abs(got - exp)) <= 1e-13
+
+
 def test_remapped_single_gates_are_exact_permutations():
     """A permutation gate under remapping moves amplitudes exactly."""
     n = 12
