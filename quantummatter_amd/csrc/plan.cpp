@@ -818,6 +818,9 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
     if (d.out_place && pass_index + 1 < replay->lanes.size()) {
       bool assigned[64] = {}, taken[64] = {};
       for (int x = 0; x < c; x++) assigned[x] = taken[x] = true;
+      // The next pass's load lanes lowest, then this pass's store lanes
+      // (measured on the 30-qubit layer circuit: 250-252 gate-layers/s against
+      // 246 the other way round).
       const uint32_t nl = replay->lanes[pass_index + 1];
       for (int k = 0; k < 3; k++) {
         const int q = (int)((nl >> (8 * k)) & 0xffu);
