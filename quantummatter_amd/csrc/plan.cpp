@@ -839,11 +839,18 @@ FusedPlan plan_fused(const std::vector<DevGate> &gates_in, int n, const FusedSha
         mstore[v] = next;
         assigned[v] = taken[next] = true;
       }
+      // The other bits fill the free index bits in the order they are loaded
+      // from, so the tile index bits that vary fastest across the grid stay
+      // on the lowest non-tile index bits for the store too.
+      int rest[64], n_rest = 0;
+      for (int v = c; v < n; v++)
+        if (!assigned[v]) rest[n_rest++] = v;
+      std::sort(rest, rest + n_rest,
+                [&](int a, int b) { return mload[phys[newlogi[a]]] < mload[phys[newlogi[b]]]; });
       int free_phys = c;
-      for (int v = c; v < n; v++) {
-        if (assigned[v]) continue;
+      for (int i = 0; i < n_rest; i++) {
         while (taken[free_phys]) free_phys++;
-        mstore[v] = free_phys;
+        mstore[rest[i]] = free_phys;
         taken[free_phys] = true;
       }
     }
