@@ -223,10 +223,11 @@ struct FusedShape {
   // permutation of the index bits above the `contiguous` low bits chosen so
   // that the next pass's thread bits contiguous..contiguous+2 (the lane bits
   // of a wave above one contiguous run) load from index bits
-  // contiguous..contiguous+2 and this pass's own store lanes go to the next
-  // three bits: each load and store instruction of a wave then covers
-  // 2^(contiguous+3) consecutive amplitudes or a few nearby runs instead of
-  // eight runs far apart.  The last pass stores in natural order.
+  // contiguous..contiguous+2, this pass's own store lanes go to the next
+  // three bits and the next pass's other tile qubits right above: each load
+  // and store instruction of a wave then covers 2^(contiguous+3) consecutive
+  // amplitudes or a few nearby runs instead of eight runs far apart, and the
+  // next pass's tiles are compact.  The last pass stores in natural order.
   int lane_layout = 0;
 };
 constexpr int kSearchNoise = 2;
