@@ -101,12 +101,14 @@ struct Ctx {
   // each with a 64 KiB LDS exchange buffer, so one tile's loads and stores
   // overlap the other's arithmetic), the low 3 index bits (or 4 when that plan
   // models faster) remapped between passes; complex64 16384-amplitude tiles,
-  // 512 threads x 32 registers (two workgroups per CU), the low 5 bits.  Over
+  // 512 threads x 32 registers (two workgroups per CU), the low 4 bits (128-byte
+  // runs; or 5 when that plan models faster: on the 30-qubit layer circuit 15
+  // passes against 17 with 5, 444-447 gate-layers/s against 436).  Over
   // six layer circuits of 26-30 qubits the complex128 shape runs 1-23 %
   // faster than 16384-amplitude tiles with one workgroup per CU (18 passes of
   // the 30-qubit 40-layer circuit against 14, each shorter); in complex64 the
   // 16384-amplitude shape is the faster (19 passes).
-  FusedShape shape[2] = {alt_low_bits(compiled_shape(13, 8, 3), 4), compiled_shape(14, 9, 5)};
+  FusedShape shape[2] = {alt_low_bits(compiled_shape(13, 8, 3), 4), alt_low_bits(compiled_shape(14, 9, 4), 5)};
   // Shapes chosen by qm_autotune, [fused 0 / compiled 1][precision][qubits];
   // K == 0: not tuned.  They take precedence over `shape` for their width.
   FusedShape tuned[2][2][35] = {};
