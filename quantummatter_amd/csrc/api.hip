@@ -299,7 +299,8 @@ double compiled_plan_cost(const FusedPlan &p) {
   for (const FusedPass &x : p.passes) t += 1.8 + 0.26 * x.transposes;
   return t;
 }
-FusedPlan plan_compiled(const std::vector<DevGate> &gates, int n, const FusedShape &shape, const std::string &key) {
+FusedPlan plan_compiled(const std::vector<DevGate> &gates, int n, const FusedShape &shape, const std::string &key,
+                        CachedPlan *out = nullptr) {
   static const int env_restarts = [] {
     const char *e = std::getenv("QM_PLAN_RESTARTS");
     return e ? std::max(0, std::atoi(e)) : -1;
@@ -332,10 +333,21 @@ FusedPlan plan_compiled(const std::vector<DevGate> &gates, int n, const FusedSha
   // The lane layout needs each next pass's load lanes: the chosen plan is
   // replayed from its trace, which holds them (same tiles, phases and gates).
   if (rec.shape.lane_layout) plan = plan_fused(gates, n, rec.shape, &rec.trace);
+  if (out) *out = rec;
   plan_cache_put(key, std::move(rec));
   return plan;
 }
 }  // namespace
+
+FusedPlan plan_compiled_trace(const std::vector<DevGate> &gates, int n, int precision, const FusedShape &shape,
+                              FusedShape *chosen, PlanTrace *trace) {
+  const std::string key = plan_key(gates, n, precision, shape, QM_SCHED_COMPILED);
+  CachedPlan hit;
+  if (!plan_cache_get(key, &hit)) (void)plan_compiled(gates, n, shape, key, &hit);
+  *chosen = hit.shape;
+  *trace = hit.trace;
+  return plan_fused(gates, n, hit.shape, &hit.trace);
+}
 
 // Runs prepared gates on a resident state with the chosen schedule.
 // Whether QM_SCHED_AUTO compiles a structure's kernels in the background on

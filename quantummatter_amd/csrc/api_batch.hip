@@ -9,9 +9,11 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "batch.hpp"
+#include "jit.hpp"
 #include "qm_internal.hpp"
 
 namespace qm {
@@ -273,7 +275,13 @@ void run_batch_launch(Ctx *c, BatchLaunch &L) {
   const BatchArgs a = A.args(L);
   int64_t grid = L.keep ? 0 : A.slots;
   if (L.timed) QM_HIP(hipEventRecord(c->ev_start, st));
-  launch_batch(c, L.precision, L.shape, a, &grid);
+  L.evolved = L.evolve && slices;
+  if (L.evolved) {
+    QM_HIP(hipMemsetAsync(a.err, 0, (size_t)n_items * sizeof(int32_t), st));
+    L.evolve(a.states, st);
+  } else {
+    launch_batch(c, L.precision, L.shape, a, &grid);
+  }
   if (slices)
     launch_pauli_slices(c, L.precision, a.states, a.items, n_items, tile_pairs, a.groups, gb, gc, n_terms, slices,
                         A.ar.at<double>(A.o_part), A.ar.at<double>(A.o_part_mass), a.numer, a.mass);
@@ -404,13 +412,148 @@ void collect_out(const BatchJob &job, const TermGroups &g, const int32_t *err, c
   out.algorithmic_bytes = (double)job.items * (double)state_bytes * (2.0 * (double)passes + finals);
 }
 
+// Batched compiled passes: the energy and gradient jobs of states of at least
+// QM_BATCH_COMPILED_QUBITS qubits (tuning knob, default 18; 0 turns it off)
+// run the compiled schedule's pass kernels over all their states at once
+// instead of the interpreting batch kernel.  Every item runs the same gates
+// but its own rotation angles: the plan is searched once for the structure,
+// each item's plan is replayed from its trace with its angles, and the pass
+// kernels (generated without the angle-dependent factored forms, so one
+// source serves every item) read a coefficient block per item, filled from
+// the op coefficients each slot copies.  The states are kept for the
+// reduction after the passes.
+static int compiled_batch_qubits() {
+  static const int q = [] {
+    const char *e = std::getenv("QM_BATCH_COMPILED_QUBITS");
+    return e ? std::atoi(e) : 18;
+  }();
+  return q;
+}
+
+static bool compiled_batch_fits(const BatchJob &job) {
+  const int q = compiled_batch_qubits();
+  if (q <= 0 || job.qubits < q || job.shots || job.measure || job.pools || job.items < 1) return false;
+  for (const BatchStep &st : job.steps)
+    if (st.damp_target >= 0 || st.gate.kind == G_NOISE || st.gate.kind == G_DAMP) return false;
+  return true;
+}
+
+// Item i's gates: the dynamic rotations resolved from its angle table.
+static std::vector<DevGate> item_gates(const BatchJob &job, int64_t i) {
+  std::vector<DevGate> g;
+  g.reserve(job.steps.size());
+  for (const BatchStep &st : job.steps) {
+    DevGate d = st.gate;
+    if (d.kind == G_PROT || d.kind == G_PROTZ) {
+      const double *a = &job.angles[(size_t)((i * job.slots + d.aux0) * 2)];
+      for (double &v : d.c) v = 0;
+      if (d.kind == G_PROT) {
+        d.kind = d.aux1;  // G_RX or G_RY: c[0] = cos, c[1] = sin of half the angle
+        d.c[0] = a[0];
+        d.c[1] = a[1];
+      } else {
+        d.kind = G_DIAG;  // Rz = diag(conj(D), D), D = (cos, sin) of half the angle
+        d.c[0] = a[0];
+        d.c[1] = -a[1];
+        d.c[6] = a[0];
+        d.c[7] = a[1];
+      }
+      d.aux0 = d.aux1 = 0;
+    }
+    g.push_back(d);
+  }
+  return g;
+}
+
+struct CompiledBatch {
+  FusedShape shape;
+  JitProgram jp;
+  std::vector<double> coef;    // per pass: [items][slots of the pass]
+  std::vector<int64_t> off;    // first coefficient of each pass's blocks
+  int passes = 0;
+};
+
+static bool prepare_compiled_batch(Ctx *c, const BatchJob &job, CompiledBatch &r) {
+  const int n = job.qubits;
+  FusedShape base = c->shape[job.precision];
+  apply_tuning_knobs(base);
+  base.lane_layout = 0;
+  base.variant = jit_default_variant(job.precision) & ~(16 | 8192);  // no factored or merged diagonal forms
+  if (n < base.K) return false;
+  FusedShape chosen;
+  PlanTrace trace;
+  FusedPlan plan = plan_compiled_trace(item_gates(job, 0), n, job.precision, base, &chosen, &trace);
+  chosen.variant = base.variant;  // a cached plan of the structure may come from another variant
+  chosen.lane_layout = 0;
+  if (plan.passes.empty() || !jit_supported(plan)) return false;
+  plan = plan_fused(item_gates(job, 0), n, chosen, &trace);
+  plan.shape.state_bits = n;
+  r.shape = plan.shape;
+  r.jp = jit_prepare(plan, job.precision, c->device);
+  r.passes = (int)plan.passes.size();
+  r.off.assign(plan.passes.size(), 0);
+  int64_t total = 0;
+  for (size_t p = 0; p < plan.passes.size(); p++) {
+    r.off[p] = total;
+    total += job.items * (int64_t)(r.jp.prov[p].size() / 2);
+  }
+  r.coef.assign((size_t)total, 0.0);
+  // Each item's plan replayed with its angles (the same passes and ops), its
+  // coefficients copied into its blocks; items split over host threads.
+  const int threads = (int)std::max<int64_t>(1, std::min<int64_t>(
+                          std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())), job.items));
+  std::vector<std::string> errs((size_t)threads);
+  auto work = [&](int w) {
+    try {
+      for (int64_t i = w; i < job.items; i += threads) {
+        const FusedPlan pi = plan_fused(item_gates(job, i), n, chosen, &trace);
+        if (pi.passes.size() != plan.passes.size()) fail(QM_ERR_DEVICE, "a batch item's plan differs");
+        for (size_t p = 0; p < pi.passes.size(); p++) {
+          const std::vector<int32_t> &pv = r.jp.prov[p];
+          const size_t ns = pv.size() / 2;
+          const std::vector<TileOp> &ops = pi.passes[p].ops;
+          if (ops.size() != plan.passes[p].ops.size()) fail(QM_ERR_DEVICE, "a batch item's plan differs");
+          double *dst = &r.coef[(size_t)(r.off[p] + i * (int64_t)ns)];
+          for (size_t k = 0; k < ns; k++) dst[k] = ops[(size_t)pv[2 * k]].c[pv[2 * k + 1]];
+        }
+      }
+    } catch (const Status &e) {
+      errs[(size_t)w] = e.msg.empty() ? "batch item planning failed" : e.msg;
+    } catch (...) {
+      errs[(size_t)w] = "batch item planning failed";
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int w = 1; w < threads; w++) pool.emplace_back(work, w);
+  work(0);
+  for (std::thread &t : pool) t.join();
+  for (const std::string &e : errs)
+    if (!e.empty()) fail(QM_ERR_DEVICE, e);
+  return true;
+}
+
 void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes, int64_t *consumed,
                bool timed, BatchOut &out) {
   BatchLaunch L;
   const TermGroups g = build_launch(job, L);
   L.timed = timed;
+  CompiledBatch cb;
+  double *d_coef = nullptr;
+  if (compiled_batch_fits(job) && prepare_compiled_batch(c, job, cb)) {
+    const int64_t bytes = (int64_t)(cb.coef.size() * sizeof(double));
+    d_coef = (double *)ctx_program(c, bytes + 256);
+    if (bytes)
+      QM_HIP(hipMemcpyAsync(d_coef, cb.coef.data(), (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    L.evolve = [&](void *states, hipStream_t st) {
+      launch_batch_basis(c, job.precision, states, job.items, 1ull << job.qubits);
+      for (int p = 0; p < cb.passes; p++)
+        jit_launch_batch(c->compute_units, job.precision, cb.shape, cb.jp, (size_t)p, d_coef + cb.off[(size_t)p],
+                         job.items, states, st);
+    };
+  }
   run_batch_launch(c, L);
-  collect_out(job, g, L.err.data(), L.numer.data(), L.mass.data(), (int)L.prog.passes.size(), out);
+  collect_out(job, g, L.err.data(), L.numer.data(), L.mass.data(),
+              L.evolved ? cb.passes : (int)L.prog.passes.size(), out);
   if (samples && job.shots) std::memcpy(samples, L.samples.data(), L.samples.size() * 8);
   if (outcomes && job.measure) std::memcpy(outcomes, L.outcomes.data(), L.outcomes.size() * 8);
   if (consumed) std::memcpy(consumed, L.consumed.data(), L.consumed.size() * 8);

@@ -702,6 +702,29 @@ __global__ void k_pauli_fold(const BatchItem *__restrict__ items, int64_t n_item
   }
 }
 
+// |0...0> in each of the consecutive states of len (a power of two) amplitudes.
+template <typename T>
+__global__ void k_batch_basis(T *__restrict__ s, uint64_t total, uint64_t len) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    T a;
+    a.re = (i & (len - 1)) == 0 ? 1 : 0;
+    a.im = 0;
+    s[i] = a;
+  }
+}
+
+void launch_batch_basis(Ctx *c, int precision, void *states, int64_t items, uint64_t len) {
+  const uint64_t total = (uint64_t)items * len;
+  const unsigned blocks = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((total + 255) / 256, (uint64_t)c->compute_units * 8));
+  if (precision == QM_C128)
+    k_batch_basis<C128><<<blocks, 256, 0, c->stream>>>(static_cast<C128 *>(states), total, len);
+  else
+    k_batch_basis<C64><<<blocks, 256, 0, c->stream>>>(static_cast<C64 *>(states), total, len);
+  QM_HIP(hipGetLastError());
+}
+
 namespace {
 
 template <typename T, int L, int R>

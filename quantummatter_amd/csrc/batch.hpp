@@ -86,6 +86,11 @@ struct BatchLaunch {
   const double *shot_u = nullptr;  // [n_shots]
   bool timed = false;
   std::function<void(const void *states, const double *mass, hipStream_t stream)> on_states;
+  // Replaces the batch kernel when the states are kept for the reduction
+  // after it (batched compiled passes): evolves every item's state at
+  // states + item.state_off from |0...0>.
+  std::function<void(void *states, hipStream_t stream)> evolve;
+  bool evolved = false;  // set by run_batch_launch: evolve ran instead of the batch kernel
   std::vector<int32_t> err;
   std::vector<double> numer, mass;
   std::vector<int64_t> samples, outcomes, consumed;
@@ -94,6 +99,9 @@ struct BatchLaunch {
 };
 
 void run_batch_launch(Ctx *c, BatchLaunch &L);
+
+// Sets each of `items` consecutive states of `len` amplitudes to |0...0>.
+void launch_batch_basis(Ctx *c, int precision, void *states, int64_t items, uint64_t len);
 
 // Largest compiled batch tile shape with K <= qubits (K = 0: none).
 FusedShape batch_shape(int precision, int qubits);

@@ -427,7 +427,7 @@ static std::vector<uint32_t> exchange_layout(const DevPass &d, int from, int to,
 }
 
 std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
-                        std::vector<double> &coef) {
+                        std::vector<double> &coef, std::vector<int32_t> *prov = nullptr) {
   const int variant = variant_of(sh, precision);
   const bool split = (variant & 1) != 0, occ3 = (variant & 2) != 0, inline_j = (variant & 4) != 0;
   const bool nontemporal = (variant & 8) != 0, factored = (variant & 16) != 0, frames = (variant & 32) != 0;
@@ -439,6 +439,11 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   const bool merge_diag = (variant & 8192) != 0 && !frames && factored;
   const DevPass &d = p.dev;
   const int L = sh.L, R = sh.R, N = 1 << R;
+  const bool batched = sh.state_bits > 0;
+  const int tile_shift = sh.state_bits - sh.K;
+  if (batched && (!tile_coef || !scalar_coef || prefetch || factored || merge_diag || d.out_place || d.store_nruns))
+    fail(QM_ERR_UNSUPPORTED, "batched compiled passes need per-tile scalar coefficients, no factored forms, "
+                             "no prefetch and in-place passes");
   const bool c128 = precision == QM_C128;
   const char *F = c128 ? "double" : "float";
   const char *T = c128 ? "qj_c128" : "qj_c64";
@@ -497,7 +502,7 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
     return e;
   };
   auto loads = [&](const char *re, const char *im, const std::string &b) {
-    o << "    { const " << T << " *src = s + ((" << b << ") | ll);\n";
+    o << "    { const " << T << " *src = " << (batched ? "sp" : "s") << " + ((" << b << ") | ll);\n";
     for (int r = 0; r < N; r++) {
       if (nontemporal)
         o << "      { const " << T << " *a = src + " << hex(d.load_reg_off[r]) << "; " << re << "[" << r
@@ -526,7 +531,11 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       // coefficient is a scalar load (waited on with lgkmcnt) instead of a
       // vector load, whose wait would also wait for the previous tile's
       // state stores still in flight.
-      o << "    unsigned long long cfa = (unsigned long long)cf_arg;\n";
+      if (batched)
+        o << "    unsigned long long cfa = (unsigned long long)(cf_arg + (tau >> " << tile_shift
+          << ") * @NCOEF@ull);\n";
+      else
+        o << "    unsigned long long cfa = (unsigned long long)cf_arg;\n";
       o << "    asm volatile(\"\" : \"+s\"(cfa));\n";
       o << "    const QJ_CONST double *cf = (const QJ_CONST double *)cfa;\n";
     } else {
@@ -534,7 +543,14 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
       o << "    asm volatile(\"\" : \"+s\"(cf));\n";
     }
   }
-  o << "    const unsigned long long base = " << base_expr("tau") << ";\n";
+  if (batched) {
+    // Batched passes: the tile index picks the state, then the tile in it.
+    o << "    " << T << " *const sp = s + ((tau >> " << tile_shift << ") << " << sh.state_bits << ");\n";
+    o << "    const unsigned long long base = " << base_expr("(tau & " + hex((1ull << tile_shift) - 1) + ")")
+      << ";\n";
+  } else {
+    o << "    const unsigned long long base = " << base_expr("tau") << ";\n";
+  }
   if (d.store_nruns) {
     std::string e = "0ull";
     for (int k = 0; k < d.store_nruns; k++)
@@ -660,6 +676,18 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
   auto cfix = [&](const double *v, int n) {
     const size_t at = coef.size();
     coef.insert(coef.end(), v, v + n);
+    if (prov) {
+      // Batched passes: every slot is a copy of an op coefficient.
+      const char *b0 = (const char *)p.ops.data(), *x = (const char *)v;
+      if (x < b0 || x >= b0 + p.ops.size() * sizeof(TileOp))
+        fail(QM_ERR_UNSUPPORTED, "batched compiled passes need coefficients copied from the ops");
+      const size_t byte = (size_t)(x - b0), op = byte / sizeof(TileOp);
+      const size_t off = (byte % sizeof(TileOp) - offsetof(TileOp, c)) / sizeof(double);
+      for (int i = 0; i < n; i++) {
+        prov->push_back((int32_t)op);
+        prov->push_back((int32_t)(off + (size_t)i));
+      }
+    }
     return at;
   };
   const std::string tmpl = std::string("<F, ") + std::to_string(N) + ", ";
@@ -1107,8 +1135,8 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
         }
     }
   }
-  o << "    { " << T << " *dst = " << (d.out_place ? "so" : "s") << " + (" << (d.store_nruns ? "sbase" : "base")
-    << " | (sl" << flips << "));\n";
+  o << "    { " << T << " *dst = " << (batched ? "sp" : d.out_place ? "so" : "s") << " + ("
+    << (d.store_nruns ? "sbase" : "base") << " | (sl" << flips << "));\n";
   const int nf = (int)rflips.size();
   for (int m = 0; nf && m < (1 << nf); m++) {
     o << "      " << T << " *const dst" << m << " = dst";
@@ -1134,7 +1162,12 @@ std::string pass_source(const FusedPass &p, const FusedShape &sh, int precision,
         << hex(d.store_reg_off[r]) << "] = a; }\n";
   }
   o << "    }\n  }\n}\n";
-  return o.str();
+  if (!batched) return o.str();
+  std::string out = o.str();
+  const std::string mark = "@NCOEF@";
+  for (size_t at = out.find(mark); at != std::string::npos; at = out.find(mark, at))
+    out.replace(at, mark.size(), std::to_string(coef.size()));
+  return out;
 }
 
 struct Compiled {
@@ -1256,6 +1289,8 @@ std::vector<std::vector<char>> compile_all(const std::vector<std::string> &srcs)
 
 }  // namespace
 
+int jit_default_variant(int precision) { return jit_variant(precision); }
+
 bool jit_supported(const FusedPlan &plan) {
   for (const FusedPass &p : plan.passes)
     for (const TileOp &op : p.ops)
@@ -1271,7 +1306,12 @@ static void plan_keys(const FusedPlan &plan, int precision, int device, JitProgr
     // Coefficient indices in the source are relative to the pass (the source,
     // and so the cache key, does not depend on where the pass sits).
     std::vector<double> local;
-    std::string src = pass_source(plan.passes[i], plan.shape, precision, local);
+    std::vector<int32_t> *prov = nullptr;
+    if (plan.shape.state_bits > 0) {
+      jp.prov.resize(plan.passes.size());
+      prov = &jp.prov[i];
+    }
+    std::string src = pass_source(plan.passes[i], plan.shape, precision, local, prov);
     jp.coef_offset.push_back((int64_t)jp.coef.size());
     jp.coef.insert(jp.coef.end(), local.begin(), local.end());
     keys[i] = std::to_string(device) + "\n" + kJitPrelude + src;
@@ -1450,6 +1490,18 @@ JitProgram jit_prepare(const FusedPlan &plan, int precision, int device) {
     jp.resident.push_back(c.map[k].resident);
   }
   return jp;
+}
+
+void jit_launch_batch(int compute_units, int precision, const FusedShape &shape, const JitProgram &jp,
+                      size_t pass, const double *d_coef, int64_t items, void *states, hipStream_t st) {
+  const uint64_t tiles = (uint64_t)items << (shape.state_bits - shape.K);
+  const size_t lds = jit_lds_bytes(precision, shape);
+  const uint64_t want = (uint64_t)compute_units * (uint64_t)jp.resident[pass];
+  const unsigned blocks = (unsigned)std::min<uint64_t>(tiles, std::max<uint64_t>(want, 1));
+  unsigned long long nt = tiles;
+  void *args[] = {&states, &d_coef, &nt};
+  QM_HIP(hipModuleLaunchKernel((hipFunction_t)jp.fns[pass], blocks, 1, 1, 1u << shape.L, 1, 1, (unsigned)lds, st,
+                               args, nullptr));
 }
 
 void jit_launch(State *s, const FusedShape &shape, const JitProgram &jp, size_t pass,

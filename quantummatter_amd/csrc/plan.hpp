@@ -229,6 +229,10 @@ struct FusedShape {
   // amplitudes or a few nearby runs instead of eight runs far apart, and the
   // next pass's tiles are compact.  The last pass stores in natural order.
   int lane_layout = 0;
+  // Batched compiled passes (> 0): the pass kernels run over consecutive
+  // states of 2^state_bits amplitudes (tile index = state << (state_bits - K)
+  // | tile), each state with its own block of the pass's coefficients.
+  int state_bits = 0;
 };
 constexpr int kSearchNoise = 2;
 

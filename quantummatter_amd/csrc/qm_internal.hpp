@@ -156,6 +156,11 @@ double *ctx_partials(Ctx *c, int64_t n);
 int64_t *ctx_ipartials(Ctx *c, int64_t n);
 double *ctx_pinned(Ctx *c, int64_t n);
 void *ctx_program(Ctx *c, int64_t bytes);
+// The compiled schedule's searched plan of a structure, through the plan cache
+// (api.hip); *chosen and *trace replay it for other coefficients.
+FusedPlan plan_compiled_trace(const std::vector<DevGate> &gates, int n, int precision, const FusedShape &shape,
+                              FusedShape *chosen, PlanTrace *trace);
+void apply_tuning_knobs(FusedShape &shape);
 // Synchronous host <-> device copies staged through two pinned chunks of the
 // context (the DMA of one chunk overlaps the host copy of the other).
 void copy_to_device(Ctx *c, void *dst, const void *src, size_t bytes);
