@@ -438,14 +438,15 @@ static bool compiled_batch_fits(const BatchJob &job) {
   return true;
 }
 
-// Item i's gates: the dynamic rotations resolved from its angle table.
-static std::vector<DevGate> item_gates(const BatchJob &job, int64_t i) {
+// Item i's gates: the dynamic rotations resolved from its angle table
+// (angles: [items][slots][2]).
+static std::vector<DevGate> item_gates(const BatchJob &job, const double *angles, int64_t i) {
   std::vector<DevGate> g;
   g.reserve(job.steps.size());
   for (const BatchStep &st : job.steps) {
     DevGate d = st.gate;
     if (d.kind == G_PROT || d.kind == G_PROTZ) {
-      const double *a = &job.angles[(size_t)((i * job.slots + d.aux0) * 2)];
+      const double *a = &angles[(size_t)((i * job.slots + d.aux0) * 2)];
       for (double &v : d.c) v = 0;
       if (d.kind == G_PROT) {
         d.kind = d.aux1;  // G_RX or G_RY: c[0] = cos, c[1] = sin of half the angle
@@ -466,55 +467,36 @@ static std::vector<DevGate> item_gates(const BatchJob &job, int64_t i) {
 }
 
 struct CompiledBatch {
-  FusedShape shape;
+  FusedShape shape;            // the kernels' shape (state_bits set)
+  FusedShape chosen;           // the planning shape, replayed per item with `trace`
+  PlanTrace trace;
   JitProgram jp;
+  std::vector<size_t> op_count;  // ops of each pass (every item's plan has the same)
   std::vector<double> coef;    // per pass: [items][slots of the pass]
   std::vector<int64_t> off;    // first coefficient of each pass's blocks
   int passes = 0;
 };
 
-static bool prepare_compiled_batch(Ctx *c, const BatchJob &job, CompiledBatch &r) {
+// Every item's coefficient blocks for the angles at `angles`: each item's plan
+// replayed with its angles (the same passes and ops), its coefficients copied
+// into its blocks; items split over host threads.
+static void fill_compiled_coef(const BatchJob &job, const double *angles, const CompiledBatch &r, double *dst) {
   const int n = job.qubits;
-  FusedShape base = c->shape[job.precision];
-  apply_tuning_knobs(base);
-  base.lane_layout = 0;
-  base.variant = jit_default_variant(job.precision) & ~(16 | 8192);  // no factored or merged diagonal forms
-  if (n < base.K) return false;
-  FusedShape chosen;
-  PlanTrace trace;
-  FusedPlan plan = plan_compiled_trace(item_gates(job, 0), n, job.precision, base, &chosen, &trace);
-  chosen.variant = base.variant;  // a cached plan of the structure may come from another variant
-  chosen.lane_layout = 0;
-  if (plan.passes.empty() || !jit_supported(plan)) return false;
-  plan = plan_fused(item_gates(job, 0), n, chosen, &trace);
-  plan.shape.state_bits = n;
-  r.shape = plan.shape;
-  r.jp = jit_prepare(plan, job.precision, c->device);
-  r.passes = (int)plan.passes.size();
-  r.off.assign(plan.passes.size(), 0);
-  int64_t total = 0;
-  for (size_t p = 0; p < plan.passes.size(); p++) {
-    r.off[p] = total;
-    total += job.items * (int64_t)(r.jp.prov[p].size() / 2);
-  }
-  r.coef.assign((size_t)total, 0.0);
-  // Each item's plan replayed with its angles (the same passes and ops), its
-  // coefficients copied into its blocks; items split over host threads.
   const int threads = (int)std::max<int64_t>(1, std::min<int64_t>(
                           std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())), job.items));
   std::vector<std::string> errs((size_t)threads);
   auto work = [&](int w) {
     try {
       for (int64_t i = w; i < job.items; i += threads) {
-        const FusedPlan pi = plan_fused(item_gates(job, i), n, chosen, &trace);
-        if (pi.passes.size() != plan.passes.size()) fail(QM_ERR_DEVICE, "a batch item's plan differs");
+        const FusedPlan pi = plan_fused(item_gates(job, angles, i), n, r.chosen, &r.trace);
+        if (pi.passes.size() != r.op_count.size()) fail(QM_ERR_DEVICE, "a batch item's plan differs");
         for (size_t p = 0; p < pi.passes.size(); p++) {
           const std::vector<int32_t> &pv = r.jp.prov[p];
           const size_t ns = pv.size() / 2;
           const std::vector<TileOp> &ops = pi.passes[p].ops;
-          if (ops.size() != plan.passes[p].ops.size()) fail(QM_ERR_DEVICE, "a batch item's plan differs");
-          double *dst = &r.coef[(size_t)(r.off[p] + i * (int64_t)ns)];
-          for (size_t k = 0; k < ns; k++) dst[k] = ops[(size_t)pv[2 * k]].c[pv[2 * k + 1]];
+          if (ops.size() != r.op_count[p]) fail(QM_ERR_DEVICE, "a batch item's plan differs");
+          double *d = dst + r.off[p] + i * (int64_t)ns;
+          for (size_t k = 0; k < ns; k++) d[k] = ops[(size_t)pv[2 * k]].c[pv[2 * k + 1]];
         }
       }
     } catch (const Status &e) {
@@ -529,7 +511,50 @@ static bool prepare_compiled_batch(Ctx *c, const BatchJob &job, CompiledBatch &r
   for (std::thread &t : pool) t.join();
   for (const std::string &e : errs)
     if (!e.empty()) fail(QM_ERR_DEVICE, e);
+}
+
+static bool prepare_compiled_batch(Ctx *c, const BatchJob &job, CompiledBatch &r) {
+  const int n = job.qubits;
+  FusedShape base = c->shape[job.precision];
+  apply_tuning_knobs(base);
+  base.lane_layout = 0;
+  base.variant = jit_default_variant(job.precision) & ~(16 | 8192);  // no factored or merged diagonal forms
+  if (n < base.K) return false;
+  const std::vector<DevGate> g0 = item_gates(job, job.angles.data(), 0);
+  FusedPlan plan = plan_compiled_trace(g0, n, job.precision, base, &r.chosen, &r.trace);
+  r.chosen.variant = base.variant;  // a cached plan of the structure may come from another variant
+  r.chosen.lane_layout = 0;
+  if (plan.passes.empty() || !jit_supported(plan)) return false;
+  plan = plan_fused(g0, n, r.chosen, &r.trace);
+  plan.shape.state_bits = n;
+  r.shape = plan.shape;
+  r.jp = jit_prepare(plan, job.precision, c->device);
+  r.passes = (int)plan.passes.size();
+  r.off.assign(plan.passes.size(), 0);
+  r.op_count.clear();
+  int64_t total = 0;
+  for (size_t p = 0; p < plan.passes.size(); p++) {
+    r.off[p] = total;
+    r.op_count.push_back(plan.passes[p].ops.size());
+    total += job.items * (int64_t)(r.jp.prov[p].size() / 2);
+  }
+  r.coef.assign((size_t)total, 0.0);
+  fill_compiled_coef(job, job.angles.data(), r, r.coef.data());
   return true;
+}
+
+// The compiled batch's stream work on the kept states: coefficients from
+// `h_coef` (pinned when under graph capture), statuses cleared, |0...0>,
+// every pass.
+static void issue_compiled_batch(Ctx *c, const BatchJob &job, const CompiledBatch &cb, const double *h_coef,
+                                 double *d_coef, int32_t *err, void *states, hipStream_t st) {
+  if (!cb.coef.empty())
+    QM_HIP(hipMemcpyAsync(d_coef, h_coef, cb.coef.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  launch_zero_i32(c, err, job.items);
+  launch_batch_basis(c, job.precision, states, job.items, 1ull << job.qubits);
+  for (int p = 0; p < cb.passes; p++)
+    jit_launch_batch(c->compute_units, job.precision, cb.shape, cb.jp, (size_t)p, d_coef + cb.off[(size_t)p],
+                     job.items, states, st);
 }
 
 void run_batch(Ctx *c, const BatchJob &job, int64_t *samples, int64_t *outcomes, int64_t *consumed,
@@ -917,6 +942,11 @@ struct GraphedBatch {
   bool tile_pairs = false;  // tile-pair reduction (pauli_slices)
   int64_t held = 0;        // arena bytes counted in the context's resident bytes
   uint64_t last_use = 0;   // program tick of the last evaluation (cache eviction)
+  // Batched compiled passes (compiled_batch_fits): coefficient blocks formed
+  // on the host from each evaluation's angles, copied from pinned memory.
+  bool compiled = false;
+  CompiledBatch cb;
+  double *h_coef = nullptr, *d_coef = nullptr;
 
   ~GraphedBatch() {
     if (c && held) c->resident_bytes -= held;
@@ -924,6 +954,8 @@ struct GraphedBatch {
     if (graph) (void)hipGraphDestroy(graph);
     if (h_angles) (void)hipHostFree(h_angles);
     if (h_out) (void)hipHostFree(h_out);
+    if (h_coef) (void)hipHostFree(h_coef);
+    if (d_coef) (void)hipFree(d_coef);
   }
 
   // The stream work of one evaluation, issued eagerly or under capture.
@@ -935,7 +967,10 @@ struct GraphedBatch {
     k_zero_counter<<<1, 64, 0, st>>>(A.ar.at<unsigned long long>(A.o_fb));
     QM_HIP(hipGetLastError());
     int64_t grid = L.keep ? 0 : A.slots;
-    launch_batch(c, L.precision, L.shape, a, &grid);
+    if (compiled)
+      issue_compiled_batch(c, job, cb, h_coef, d_coef, a.err, a.states, st);
+    else
+      launch_batch(c, L.precision, L.shape, a, &grid);
     const int64_t n = (int64_t)L.items.size();
     if (slices)
       launch_pauli_slices(c, L.precision, a.states, a.items, n, tile_pairs, a.groups, group_begin, group_count,
@@ -959,6 +994,17 @@ struct GraphedBatch {
     // budget checks see it.
     held = (int64_t)A.ar.size;
     c->resident_bytes += held;
+    if (slices && compiled_batch_fits(job) && prepare_compiled_batch(c, job, cb)) {
+      const size_t bytes = std::max<size_t>(cb.coef.size(), 1) * sizeof(double);
+      if (hipHostMalloc((void **)&h_coef, bytes) != hipSuccess || hipMalloc((void **)&d_coef, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(QM_ERR_BUDGET, "memory for a graphed batch's coefficients is not available");
+      }
+      std::memcpy(h_coef, cb.coef.data(), cb.coef.size() * sizeof(double));
+      held += (int64_t)bytes;
+      c->resident_bytes += (int64_t)bytes;
+      compiled = true;
+    }
     const int64_t n = (int64_t)L.items.size();
     if (hipHostMalloc((void **)&h_angles, std::max<size_t>(L.angles.size(), 1) * 8) != hipSuccess ||
         (off_numer = ((size_t)n * 4 + 7) & ~(size_t)7, off_mass = off_numer + (size_t)L.n_numer * 8,
@@ -981,13 +1027,15 @@ struct GraphedBatch {
   // Replays with the angles already in h_angles; fills `out` like run_batch.
   void replay(bool timed, BatchOut &out) {
     hipStream_t st = c->stream;
+    if (compiled) fill_compiled_coef(job, h_angles, cb, h_coef);
     if (timed) QM_HIP(hipEventRecord(c->ev_start, st));
     QM_HIP(hipGraphLaunch(exec, st));
     if (timed) QM_HIP(hipEventRecord(c->ev_stop, st));
     QM_HIP(hipStreamSynchronize(st));
     collect_out(job, g, reinterpret_cast<const int32_t *>(h_out),
                 reinterpret_cast<const double *>(h_out + off_numer),
-                reinterpret_cast<const double *>(h_out + off_mass), (int)L.prog.passes.size(), out);
+                reinterpret_cast<const double *>(h_out + off_mass),
+                compiled ? cb.passes : (int)L.prog.passes.size(), out);
     out.fallbacks = (int64_t) * reinterpret_cast<const unsigned long long *>(h_out + off_fb);
     if (timed) {
       float ms = 0;

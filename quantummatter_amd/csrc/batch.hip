@@ -714,6 +714,17 @@ __global__ void k_batch_basis(T *__restrict__ s, uint64_t total, uint64_t len) {
   }
 }
 
+__global__ void k_zero_i32(int32_t *__restrict__ v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = 0;
+}
+
+void launch_zero_i32(Ctx *c, int32_t *v, int64_t n) {
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024));
+  k_zero_i32<<<blocks, 256, 0, c->stream>>>(v, n);
+  QM_HIP(hipGetLastError());
+}
+
 void launch_batch_basis(Ctx *c, int precision, void *states, int64_t items, uint64_t len) {
   const uint64_t total = (uint64_t)items * len;
   const unsigned blocks = (unsigned)std::max<uint64_t>(

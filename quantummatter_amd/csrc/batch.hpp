@@ -100,6 +100,8 @@ struct BatchLaunch {
 
 void run_batch_launch(Ctx *c, BatchLaunch &L);
 
+// Zeroes n 32-bit values (a kernel, so it can be captured into a graph).
+void launch_zero_i32(Ctx *c, int32_t *v, int64_t n);
 // Sets each of `items` consecutive states of `len` amplitudes to |0...0>.
 void launch_batch_basis(Ctx *c, int precision, void *states, int64_t items, uint64_t len);
 
