@@ -413,7 +413,8 @@ void collect_out(const BatchJob &job, const TermGroups &g, const int32_t *err, c
 }
 
 // Batched compiled passes: the energy and gradient jobs of states of at least
-// QM_BATCH_COMPILED_QUBITS qubits (tuning knob, default 18; 0 turns it off)
+// QM_BATCH_COMPILED_QUBITS qubits (tuning knob, default 15, where the states
+// are kept for the reduction after the passes; 0 turns it off)
 // run the compiled schedule's pass kernels over all their states at once
 // instead of the interpreting batch kernel.  Every item runs the same gates
 // but its own rotation angles: the plan is searched once for the structure,
@@ -425,7 +426,7 @@ void collect_out(const BatchJob &job, const TermGroups &g, const int32_t *err, c
 static int compiled_batch_qubits() {
   static const int q = [] {
     const char *e = std::getenv("QM_BATCH_COMPILED_QUBITS");
-    return e ? std::atoi(e) : 18;
+    return e ? std::atoi(e) : 15;
   }();
   return q;
 }
